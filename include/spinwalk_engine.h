@@ -1,0 +1,216 @@
+/*
+ * spinwalk_engine.h -- C ABI of the MI355X (gfx950) SpinWalk simulation engine.
+ *
+ * The drop-in boundary for the `spinwalk sim -c config.ini` path of the reference
+ * (aghaeifar/SpinWalk v1.21.0).  The reference has no plugin API; its seams are
+ *   - the kernel seam  cu_sim(param, param_uvec, pFieldMap, pMask, M0, XYZ0, M1, XYZ1, T)
+ *                      (src/sim/kernels.cuh:24, src/sim/kernels.cu:55-63), launched by
+ *                      monte_carlo::run per scale (src/sim/monte_carlo.cu:324-333);
+ *   - the upload seam  "move to GPU memory" (src/sim/monte_carlo.cu:247-262) and
+ *                      parameters_dvec::copy_from_host (src/sim/simulation_parameters.cuh:74-90);
+ *   - the driver seam  monte_carlo::run(config) / sim::handler::execute
+ *                      (src/sim/monte_carlo.cu:199-355, src/sim/handler.cu:8-17).
+ * Each entry point below names the reference interface it replaces.
+ *
+ * Conventions: plain C types only; every call returns an int status (SW_OK == 0,
+ * negative on error) and the engine keeps a message for sw_last_error().  Device
+ * pointers are plain `void*`/typed pointers into HBM; `stream` is a hipStream_t
+ * passed as `void*` (NULL = the engine's own stream).  One engine per host thread.
+ */
+#ifndef SPINWALK_ENGINE_H
+#define SPINWALK_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SW_API __attribute__((visibility("default")))
+
+/* ---- status codes --------------------------------------------------------- */
+#define SW_OK               0
+#define SW_E_INVALID       -1   /* bad argument / inconsistent sizes                 */
+#define SW_E_HIP           -2   /* HIP runtime error                                 */
+#define SW_E_NOMEM         -3   /* device memory check failed (device_helper.cu:78-101) */
+#define SW_E_REPLAY        -4   /* a replayed spin ran out of recorded draws         */
+#define SW_E_CONFIG        -5   /* config file rejected (config_reader.cpp:195-324) */
+#define SW_E_IO            -6   /* phantom/output file error                         */
+#define SW_E_NODEVICE      -7   /* no usable gfx950 device                           */
+
+/* ---- enums ------------------------------------------------------------------ */
+#define SW_RNG_PHILOX       0   /* Philox4x32-10 keyed by (seed, global spin)        */
+#define SW_RNG_REPLAY       1   /* per-spin draw streams recorded on the host        */
+
+#define SW_PREC_F32         0   /* reference arithmetic: FP32 phase and M            */
+#define SW_PREC_F32_KAHAN   1   /* FP32 M, Kahan-compensated FP32 phase              */
+#define SW_PREC_F64         2   /* FP64 field, phase and M (the FP64 oracle path)    */
+
+#define SW_FIELD_NONE       0
+#define SW_FIELD_F32        1   /* tesla per 1 T, float32 (reference layout)         */
+#define SW_FIELD_F16        2   /* float16 x per-phantom scale                       */
+#define SW_FIELD_BF16       3   /* bfloat16 x per-phantom scale                      */
+
+#define SW_DTYPE_F32        0
+#define SW_DTYPE_F64        1
+
+typedef struct sw_engine   sw_engine;
+typedef struct sw_phantom  sw_phantom;
+typedef struct sw_sequence sw_sequence;
+
+/* Mirror of `parameters` (src/sim/simulation_parameters.cuh:176-246), the POD the
+ * reference passes by value to cu_sim, as it stands for one scale. */
+typedef struct sw_params {
+    double   fov[3];                  /* metres (FoV-scaled for WHAT_TO_SCALE=0)     */
+    uint64_t phantom_size[3];
+    int64_t  matrix_length;
+    float    B0, c, s;                /* c, s = cos/sin(RF_FA[0]) from prepare()     */
+    float    linear_phase_cycling, quadratic_phase_cycling;
+    int32_t  timestep_us, TR_us, n_dummy_scan;
+    uint32_t n_spins, n_timepoints, n_substrate, n_scales;
+    uint64_t seed, max_iterations;
+    int32_t  enCrossFOV, enRecordTrajectory, fieldmap_exist;
+} sw_params;
+
+/* Mirror of `parameters_hvec` (src/sim/simulation_parameters.cuh:39-65) as host
+ * views.  Times are time-step indices (after config_reader::timing_scale) and
+ * diffusivity is already converted by parameters::prepare (metres/axis/step). */
+typedef struct sw_seq {
+    const double  *diffusivity;                      /* [n_substrate]                */
+    const float   *T1_ms, *T2_ms;                    /* [n_substrate]                */
+    const float   *pXY;                              /* [n_substrate^2]              */
+    uint32_t       n_substrate;
+    const float   *RF_FA_deg, *RF_PH_deg; const int32_t *RF_t; uint32_t n_rf;
+    const int32_t *TE_t; uint32_t n_te;
+    const float   *dephasing_deg; const int32_t *dephasing_t; uint32_t n_dephasing;
+    const float   *gradientX_mTm, *gradientY_mTm, *gradientZ_mTm;
+    const int32_t *gradient_t; uint32_t n_gradient;
+} sw_seq;
+
+/* Per-launch options that the reference expresses through host-side copies. */
+typedef struct sw_walk_opts {
+    uint32_t spin_offset;      /* global index of local spin 0 (RNG key, dephasing)   */
+    uint32_t n_local;          /* spins in this launch                                 */
+    float    xyz_scale;        /* XYZ0 * s in float (monte_carlo.cu:278); 1 = none     */
+    float    gradient_scale;   /* G * s in float (monte_carlo.cu:288-290); 1 = none    */
+    int32_t  rng;              /* SW_RNG_*                                              */
+    int32_t  precision;        /* SW_PREC_*; SW_PREC_F64 writes M1 as double          */
+    int32_t  field_format;     /* which phantom copy to gather (SW_FIELD_*)            */
+    int32_t  spatial_order;    /* 1 = visit spins in start-voxel brick order           */
+    const float *replay_normals;  uint64_t replay_normal_stride;   /* device, [n_local][stride] */
+    const float *replay_uniforms; uint64_t replay_uniform_stride;  /* device, [n_local][stride] */
+} sw_walk_opts;
+
+/* Per-launch counters returned by the walk. */
+typedef struct sw_walk_stats {
+    uint64_t n_lost;            /* spins abandoned (out of range or stuck)             */
+    uint64_t n_replay_exhausted;/* replayed spins that ran out of draws                */
+    uint64_t n_steps;           /* accepted spin-steps (dummy scans included)          */
+    uint64_t n_attempts;        /* attempted steps (accepted + rejected)               */
+    double   kernel_ms;         /* walk-kernel time from HIP events                    */
+} sw_walk_stats;
+
+/* ---- engine ----------------------------------------------------------------- */
+/* monte_carlo::monte_carlo(gpu_disabled, device_id) device selection
+ * (src/sim/monte_carlo.cu:33-55) + check_CUDA (src/sim/device_helper.cu:18-30). */
+SW_API int  sw_engine_create(int device_id, sw_engine **out);
+SW_API void sw_engine_destroy(sw_engine *eng);
+SW_API const char *sw_last_error(const sw_engine *eng);
+/* print_device_info (src/sim/device_helper.cu:47-76): writes a text summary. */
+SW_API int  sw_device_info(int device_id, char *buf, size_t buflen);
+/* get_device_count (src/sim/device_helper.cu:32-45) */
+SW_API int  sw_device_count(void);
+SW_API const char *sw_version(void);
+
+/* ---- host transforms -------------------------------------------------------- */
+/* parameters::prepare (src/sim/simulation_parameters.cuh:227-245): c, s,
+ * n_timepoints, diffusivity -> metres per axis per step (in place), automatic
+ * dummy-scan count.  SEED == 0 draws from std::random_device as the reference does. */
+SW_API int   sw_params_prepare(sw_params *p, double *diffusivity, float RF_FA0_deg, float T1_0_ms);
+/* config_reader::timing_scale (src/sim/config_reader.cpp:39-46), in place. */
+SW_API void  sw_timing_scale(int32_t *times_us, uint32_t n, int32_t timestep_us);
+/* T -> degrees per step factor (src/sim/monte_carlo.cu:241). */
+SW_API float sw_tesla2deg(float B0, int32_t timestep_us);
+/* initialize_position, random branch (src/sim/monte_carlo.cu:141-151). */
+SW_API void  sw_init_positions(uint64_t seed, const float fov_m[3], uint32_t n_spins, float *XYZ0);
+/* The reference CPU path's draw streams for spins [spin_begin, spin_begin+n):
+ * mt19937(seed+spin) discarded by seed+spin feeding normal_distribution<float>
+ * and uniform_real_distribution<float> (src/sim/kernels.cu:82-88).  Row i holds
+ * spin_begin+i.  n_threads host threads. */
+SW_API int   sw_replay_fill(uint64_t seed, uint32_t spin_begin, uint32_t n,
+                            float *normals, uint64_t normal_stride,
+                            float *uniforms, uint64_t uniform_stride, int n_threads);
+
+/* ---- phantom (read_phantom + fieldmap upload, monte_carlo.cu:98-123, 247-262) */
+/* mask: uint8 [nx*ny*nz] row-major, z fastest.  field_T: float32 tesla per 1 T of
+ * B0, same layout, or NULL.  formats: bitmask of (1<<SW_FIELD_F32) |
+ * (1<<SW_FIELD_F16) | (1<<SW_FIELD_BF16) copies to keep in HBM.  Host pointers;
+ * the call is synchronous.  Fails with SW_E_NOMEM when the copies do not fit. */
+SW_API int  sw_phantom_create(sw_engine *eng, const uint8_t *mask, const float *field_T,
+                              const uint64_t dims[3], const float fov_m[3], uint32_t formats,
+                              sw_phantom **out);
+/* Same, from device-resident arrays (copied into engine-owned HBM). */
+SW_API int  sw_phantom_create_device(sw_engine *eng, const uint8_t *d_mask, const float *d_field_T,
+                                     const uint64_t dims[3], const float fov_m[3], uint32_t formats,
+                                     sw_phantom **out);
+SW_API void sw_phantom_destroy(sw_phantom *ph);
+/* max(mask)+1 (src/sim/monte_carlo.cu:113) and the half-precision scale in use. */
+SW_API int  sw_phantom_info(const sw_phantom *ph, uint32_t *n_labels, float *half_scale);
+
+/* ---- sequence tables (parameters_dvec::copy_from_host) ---------------------- */
+SW_API int  sw_sequence_create(sw_engine *eng, const sw_params *p, const sw_seq *q, sw_sequence **out);
+SW_API void sw_sequence_destroy(sw_sequence *seq);
+
+/* ---- the kernel seam: cu_sim (src/sim/kernels.cu:55-63) ---------------------
+ * Device pointers, asynchronous on `stream`.  M0, XYZ0: float [n_local][3];
+ * M1: [n_local][n_TE][3] float (or double for SW_PREC_F64); T: uint8
+ * [n_local][n_TE]; XYZ1: float [n_local][traj][3] with traj = (n_dummy+1)*
+ * n_timepoints when enRecordTrajectory else 1.  Entries of M1/T of a spin that
+ * is abandoned before an echo keep their prior contents (the reference's G9).
+ * stats (nullable) is filled after an internal stream sync. */
+SW_API int  sw_walk_device(sw_engine *eng, const sw_phantom *ph, const sw_sequence *seq,
+                           const sw_params *p, const sw_walk_opts *o,
+                           const float *d_M0, const float *d_XYZ0,
+                           void *d_M1, float *d_XYZ1, uint8_t *d_T,
+                           void *stream, sw_walk_stats *stats);
+
+/* Host-pointer convenience with the reference's semantics: M1/T/XYZ1 are
+ * uploaded first (so untouched entries keep the caller's values), the walk runs,
+ * results are copied back; synchronous. */
+SW_API int  sw_walk_host(sw_engine *eng, const sw_phantom *ph, const sw_sequence *seq,
+                         const sw_params *p, const sw_walk_opts *o,
+                         const float *M0, const float *XYZ0,
+                         void *M1, float *XYZ1, uint8_t *T,
+                         const float *replay_normals, const float *replay_uniforms,
+                         sw_walk_stats *stats);
+
+/* ---- echo reduction (demo/spinwalk_bold.ipynb:733-736), FP64 ----------------
+ * out[(te*n_sub + label)*3 + {0,1,2}] = {sum Mx, sum My, count} over spins with
+ * T[spin][te] == label, summed in float64 in a fixed order (deterministic). */
+SW_API int  sw_echo_reduce_device(sw_engine *eng, const void *d_M1, int m1_dtype, const uint8_t *d_T,
+                                  uint32_t n_spins, uint32_t n_te, uint32_t n_sub,
+                                  double *d_out, void *stream);
+
+/* ---- utilities (no counterpart in the reference) ----------------------------
+ * The production draws of spins [spin0, spin0+n_spins), attempts [0, n_att):
+ * d_out[(i*n_att + k)*4 + {0,1,2,3}] = {x, y, z, u} (float, device).  Used to test
+ * the generator's statistics; the walk draws exactly these values. */
+SW_API int  sw_philox_draws_device(sw_engine *eng, uint64_t seed, uint32_t spin0, uint32_t n_spins,
+                                   uint32_t n_att, float *d_out, void *stream);
+/* Waits for `stream` (NULL: the engine's stream). */
+SW_API int  sw_stream_sync(sw_engine *eng, void *stream);
+
+/* ---- driver: monte_carlo::run (src/sim/monte_carlo.cu:199-355) --------------
+ * Implemented in libspinwalk_host.so (C++ host side: INI config, HDF5 I/O).
+ * Declared here so that a caller binds one header. */
+SW_API int  sw_run_config(sw_engine *eng, const char *config_path);
+/* sim::handler::execute (src/sim/handler.cu:8-17).  use_cpu != 0 is rejected with
+ * SW_E_INVALID: this engine has no CPU path (the reference's `-p` switch). */
+SW_API int  sw_sim_execute(int use_cpu, uint32_t device_id, const char *const *config_files,
+                           uint32_t n_files);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPINWALK_ENGINE_H */

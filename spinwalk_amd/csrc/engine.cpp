@@ -1,0 +1,726 @@
+// engine.cpp -- implementation of the C ABI declared in include/spinwalk_engine.h.
+//
+// Host-side orchestration of the HIP walk: device selection and memory checks
+// (the reference's monte_carlo constructor and device_helper), phantom and
+// sequence upload (monte_carlo.cu:247-262, simulation_parameters.cuh:74-90), the
+// per-sequence precomputation of event trigonometry, and the launch of the walk
+// (monte_carlo.cu:324-333).  Compiled by g++ against the HIP runtime API.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <math.h>
+#include <memory>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/spinwalk_engine.h"
+#include "walk_args.hpp"
+#include "walk_launch.hpp"
+
+#define SW_VERSION_STRING "spinwalk-mi355x 0.1.0 (drop-in for SpinWalk 1.21.0 sim)"
+
+namespace {
+
+constexpr double kDeg2Rad = 0.0174532925199433;   // simulation_parameters.cuh:24
+constexpr double kRad2Deg = 57.2957795130823;     // simulation_parameters.cuh:25
+constexpr double kGamma   = 267515315.;           // definitions.h:20
+
+thread_local std::string g_tls_error;
+
+}  // namespace
+
+struct sw_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    unsigned long long *d_counters = nullptr;   // 4 x u64
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // scratch reused across calls
+    void *d_sort = nullptr; size_t sort_bytes = 0;
+    uint32_t *d_order = nullptr; size_t order_cap = 0;
+    double *d_partial = nullptr; size_t partial_cap = 0;
+};
+
+struct sw_phantom {
+    sw_engine *eng = nullptr;
+    int device = 0;
+    uint64_t dims[3] = {0, 0, 0};
+    float fov[3] = {0, 0, 0};
+    uint64_t n = 0;
+    uint8_t *mask = nullptr;
+    float *field = nullptr;          // tesla per 1 T
+    uint16_t *field_f16 = nullptr;
+    uint16_t *field_bf16 = nullptr;
+    float half_scale = 1.f;
+    uint32_t n_labels = 0;
+    bool has_field = false;
+};
+
+struct sw_sequence {
+    sw_engine *eng = nullptr;
+    int device = 0;
+    uint32_t n_sub = 0, n_rf = 0, n_te = 0, n_deph = 0, n_grad = 0;
+    double *diffusivity = nullptr;
+    float *pXY = nullptr, *T1s = nullptr, *T2s = nullptr;
+    double *T1d = nullptr, *T2d = nullptr;
+    sw::RfPulse *rf = nullptr;
+    int32_t *TE_t = nullptr, *deph_t = nullptr, *grad_t = nullptr;
+    float *deph_deg = nullptr, *gx = nullptr, *gy = nullptr, *gz = nullptr;
+    // excitation table cache, keyed by the parameters it depends on
+    sw::Excitation *exc = nullptr;
+    uint32_t exc_cap = 0;
+    bool exc_valid = false;
+    float key_lin = 0, key_quad = 0; int32_t key_ndummy = -1;
+    float RF_FA0 = 0, RF_PH0 = 0;
+};
+
+namespace {
+
+int fail(sw_engine *e, int code, const std::string &msg) {
+    if (e) e->err = msg; else g_tls_error = msg;
+    return code;
+}
+
+#define SW_HIP(eng, expr)                                                                      \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return fail((eng), SW_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));  \
+    } while (0)
+
+template <class T>
+hipError_t dev_upload(T **dst, const T *src, size_t n, hipStream_t s) {
+    *dst = nullptr;
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc((void **)dst, n * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (src) return hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, s);
+    return hipSuccess;
+}
+
+void dev_free(void *p) { if (p) (void)hipFree(p); }
+
+// xrot_withphase special cases (src/sim/kernels.cuh:162-181)
+int32_t rot_kind_f(float ph) {
+    if (ph == 0.0f) return sw::ROT_X;
+    if (ph == 180.0) return sw::ROT_XNEG;
+    if (ph == 90.0) return sw::ROT_Y;
+    if (ph == -90.0 || ph == 270.0) return sw::ROT_YNEG;
+    return sw::ROT_GENERAL;
+}
+int32_t rot_kind_d(double ph) {
+    if (ph == 0.0) return sw::ROT_X;
+    if (ph == 180.0) return sw::ROT_XNEG;
+    if (ph == 90.0) return sw::ROT_Y;
+    if (ph == -90.0 || ph == 270.0) return sw::ROT_YNEG;
+    return sw::ROT_GENERAL;
+}
+
+// Excitation phase of dummy scan d (src/sim/kernels.cu:110-114), float path.
+float exc_phase_f(float rf_phase, uint32_t d, float lin, float quad) {
+    float v = (float)((double)(rf_phase + d * lin) + d * (d + 1) / 2.0 * quad);
+    while (v > 360.0) v = (float)(v - 360.0);
+    while (v < 0) v = (float)(v + 360.0);
+    return v;
+}
+double exc_phase_d(float rf_phase, uint32_t d, float lin, float quad) {
+    double v = (double)rf_phase + (double)d * lin + d * (d + 1) / 2.0 * quad;
+    while (v > 360.0) v -= 360.0;
+    while (v < 0) v += 360.0;
+    return v;
+}
+
+int ensure_exc(sw_engine *e, sw_sequence *q, const sw_params *p, hipStream_t s) {
+    if (q->exc_valid && q->key_lin == p->linear_phase_cycling && q->key_quad == p->quadratic_phase_cycling &&
+        q->key_ndummy == p->n_dummy_scan)
+        return SW_OK;
+    if (p->n_dummy_scan < 0) return fail(e, SW_E_INVALID, "n_dummy_scan < 0: call sw_params_prepare first");
+    const uint32_t n = (uint32_t)p->n_dummy_scan + 1;
+    std::vector<sw::Excitation> h(n);
+    for (uint32_t d = 0; d < n; d++) {
+        const float pf = exc_phase_f(q->RF_PH0, d, p->linear_phase_cycling, p->quadratic_phase_cycling);
+        const double pd = exc_phase_d(q->RF_PH0, d, p->linear_phase_cycling, p->quadratic_phase_cycling);
+        h[d].kind = rot_kind_f(pf);
+        h[d].kind_d = rot_kind_d(pd);
+        h[d].sp = sinf((float)(pf * kDeg2Rad));
+        h[d].cp = cosf((float)(pf * kDeg2Rad));
+        h[d].spd = sin(pd * kDeg2Rad);
+        h[d].cpd = cos(pd * kDeg2Rad);
+    }
+    if (n > q->exc_cap) {
+        SW_HIP(e, hipStreamSynchronize(s));
+        dev_free(q->exc);
+        q->exc = nullptr;
+        SW_HIP(e, hipMalloc((void **)&q->exc, n * sizeof(sw::Excitation)));
+        q->exc_cap = n;
+    }
+    SW_HIP(e, hipMemcpyAsync(q->exc, h.data(), n * sizeof(sw::Excitation), hipMemcpyHostToDevice, s));
+    SW_HIP(e, hipStreamSynchronize(s));   // h goes out of scope
+    q->exc_valid = true;
+    q->key_lin = p->linear_phase_cycling;
+    q->key_quad = p->quadratic_phase_cycling;
+    q->key_ndummy = p->n_dummy_scan;
+    return SW_OK;
+}
+
+int check_free_memory(sw_engine *e, size_t bytes) {   // device_helper.cu:78-101
+    size_t fr = 0, tot = 0;
+    SW_HIP(e, hipMemGetInfo(&fr, &tot));
+    if (bytes > fr) {
+        char b[256];
+        snprintf(b, sizeof b, "not enough device memory: need %zu MB, free %zu MB", bytes >> 20, fr >> 20);
+        return fail(e, SW_E_NOMEM, b);
+    }
+    return SW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+SW_API const char *sw_version(void) { return SW_VERSION_STRING; }
+
+SW_API int sw_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+SW_API int sw_device_info(int device_id, char *buf, size_t buflen) {
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, device_id) != hipSuccess) return SW_E_NODEVICE;
+    size_t fr = 0, tot = 0;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(device_id);
+    (void)hipMemGetInfo(&fr, &tot);
+    (void)hipSetDevice(cur);
+    if (buf && buflen)
+        snprintf(buf, buflen,
+                 "Device %d: %s (%s)\n  compute units: %d\n  clock: %d MHz\n  global memory: %zu MB "
+                 "(free %zu MB)\n  L2 cache: %d KB\n  wavefront: %d\n",
+                 device_id, pr.name, pr.gcnArchName, pr.multiProcessorCount, pr.clockRate / 1000,
+                 (size_t)pr.totalGlobalMem >> 20, fr >> 20, pr.l2CacheSize >> 10, pr.warpSize);
+    return SW_OK;
+}
+
+SW_API int sw_engine_create(int device_id, sw_engine **out) {
+    if (!out) return SW_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(nullptr, SW_E_NODEVICE, "no HIP device found");
+    if (device_id < 0 || device_id >= n)
+        return fail(nullptr, SW_E_NODEVICE, "device id " + std::to_string(device_id) + " is not available");
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, device_id) != hipSuccess)
+        return fail(nullptr, SW_E_NODEVICE, "hipGetDeviceProperties failed");
+    if (std::strncmp(pr.gcnArchName, "gfx950", 6) != 0)
+        return fail(nullptr, SW_E_NODEVICE, std::string("device is ") + pr.gcnArchName + ", this build targets gfx950");
+    std::unique_ptr<sw_engine> e(new sw_engine);
+    e->device = device_id;
+    if (hipSetDevice(device_id) != hipSuccess) return fail(nullptr, SW_E_HIP, "hipSetDevice failed");
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(nullptr, SW_E_HIP, "hipStreamCreate failed");
+    if (hipMalloc((void **)&e->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess)
+        return fail(nullptr, SW_E_HIP, "hipMalloc counters failed");
+    if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
+        return fail(nullptr, SW_E_HIP, "hipEventCreate failed");
+    *out = e.release();
+    return SW_OK;
+}
+
+SW_API void sw_engine_destroy(sw_engine *e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    dev_free(e->d_counters);
+    dev_free(e->d_sort);
+    dev_free(e->d_order);
+    dev_free(e->d_partial);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+SW_API const char *sw_last_error(const sw_engine *e) {
+    return e ? e->err.c_str() : g_tls_error.c_str();
+}
+
+// ---------------------------------------------------------------------------
+// host transforms
+// ---------------------------------------------------------------------------
+SW_API int sw_params_prepare(sw_params *p, double *diffusivity, float RF_FA0_deg, float T1_0_ms) {
+    if (!p || (p->n_substrate && !diffusivity) || p->timestep_us <= 0) return SW_E_INVALID;
+    p->c = cosf((float)(RF_FA0_deg * kDeg2Rad));                 // simulation_parameters.cuh:229
+    p->s = sinf((float)(RF_FA0_deg * kDeg2Rad));                 // :230
+    p->n_timepoints = (uint32_t)(p->TR_us / p->timestep_us);     // :231
+    if (p->seed == 0) p->seed = std::random_device{}();          // :233-234
+    for (uint32_t i = 0; i < p->n_substrate; i++)                // :236-237
+        diffusivity[i] = 1e-3 * sqrt(2. * diffusivity[i] * p->timestep_us);
+    if (p->n_dummy_scan < 0)                                     // :239-242
+        p->n_dummy_scan = (int32_t)(5.0 * T1_0_ms / float(p->TR_us * 1e-3));
+    return SW_OK;
+}
+
+SW_API void sw_timing_scale(int32_t *v, uint32_t n, int32_t dt) {   // config_reader.cpp:39-46
+    if (!v || dt == 0) return;
+    for (uint32_t i = 0; i < n; i++) v[i] = v[i] / dt;
+}
+
+SW_API float sw_tesla2deg(float B0, int32_t dt) {                 // monte_carlo.cu:241
+    return (float)(B0 * dt * 1e-6 * kGamma * kRad2Deg);
+}
+
+SW_API void sw_init_positions(uint64_t seed, const float fov[3], uint32_t n_spins, float *XYZ0) {
+    std::mt19937 gen((std::mt19937::result_type)seed);           // monte_carlo.cu:142-151
+    std::uniform_real_distribution<float> dx(0.01 * fov[0], 0.99 * fov[0]);
+    std::uniform_real_distribution<float> dy(0.01 * fov[1], 0.99 * fov[1]);
+    std::uniform_real_distribution<float> dz(0.01 * fov[2], 0.99 * fov[2]);
+    for (uint32_t i = 0; i < n_spins; i++) {
+        XYZ0[3 * (size_t)i + 0] = dx(gen);
+        XYZ0[3 * (size_t)i + 1] = dy(gen);
+        XYZ0[3 * (size_t)i + 2] = dz(gen);
+    }
+}
+
+SW_API int sw_replay_fill(uint64_t seed, uint32_t spin_begin, uint32_t n, float *normals, uint64_t ns,
+                          float *uniforms, uint64_t us, int n_threads) {
+    if ((ns && !normals) || (us && !uniforms)) return SW_E_INVALID;
+    n_threads = std::max(1, std::min(n_threads, 256));
+    auto work = [&](uint32_t i0, uint32_t i1) {
+        for (uint32_t i = i0; i < i1; i++) {
+            const uint64_t sp = seed + spin_begin + i;
+            // kernels.cu:82-88: two generators seeded seed+spin, each discarded by seed+spin
+            std::mt19937 gr((std::mt19937::result_type)sp), gu((std::mt19937::result_type)sp);
+            gr.discard(sp);
+            gu.discard(sp);
+            std::normal_distribution<float> nd(0.f, 1.0f);
+            std::uniform_real_distribution<float> ud(0.f, 1.f);
+            float *nr = normals + (uint64_t)i * ns;
+            for (uint64_t k = 0; k < ns; k++) nr[k] = nd(gr);
+            float *ur = uniforms + (uint64_t)i * us;
+            for (uint64_t k = 0; k < us; k++) ur[k] = ud(gu);
+        }
+    };
+    if (n_threads == 1 || n < 2) { work(0, n); return SW_OK; }
+    std::vector<std::thread> th;
+    const uint32_t chunk = (n + n_threads - 1) / n_threads;
+    for (int t = 0; t < n_threads; t++) {
+        const uint32_t a = t * chunk, b = std::min(n, a + chunk);
+        if (a < b) th.emplace_back(work, a, b);
+    }
+    for (auto &t : th) t.join();
+    return SW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// phantom
+// ---------------------------------------------------------------------------
+static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
+    hipStream_t s = e->stream;
+    uint32_t *d_stats = nullptr;
+    SW_HIP(e, hipMalloc((void **)&d_stats, 2 * sizeof(uint32_t)));
+    SW_HIP(e, hipMemsetAsync(d_stats, 0, 2 * sizeof(uint32_t), s));
+    SW_HIP(e, sw::launch_phantom_stats(ph->mask, ph->field, ph->n, d_stats, s));
+    uint32_t st[2] = {0, 0};
+    SW_HIP(e, hipMemcpyAsync(st, d_stats, sizeof st, hipMemcpyDeviceToHost, s));
+    SW_HIP(e, hipStreamSynchronize(s));
+    dev_free(d_stats);
+    ph->n_labels = st[0] + 1;                                    // monte_carlo.cu:113
+    float fmax;
+    std::memcpy(&fmax, &st[1], sizeof fmax);
+    // power-of-two scale mapping max|field| to about 2^14
+    int ex = 0;
+    if (fmax > 0 && std::isfinite(fmax)) { std::frexp(fmax, &ex); ph->half_scale = std::ldexp(1.0f, ex - 15); }
+    else ph->half_scale = 1.f;
+    if (ph->has_field && (formats & ((1u << SW_FIELD_F16) | (1u << SW_FIELD_BF16)))) {
+        if (formats & (1u << SW_FIELD_F16)) {
+            SW_HIP(e, hipMalloc((void **)&ph->field_f16, ph->n * sizeof(uint16_t)));
+            SW_HIP(e, sw::launch_field_to_half(ph->field, ph->n, 1.0f / ph->half_scale, 0, ph->field_f16, s));
+        }
+        if (formats & (1u << SW_FIELD_BF16)) {
+            SW_HIP(e, hipMalloc((void **)&ph->field_bf16, ph->n * sizeof(uint16_t)));
+            SW_HIP(e, sw::launch_field_to_half(ph->field, ph->n, 1.0f / ph->half_scale, 1, ph->field_bf16, s));
+        }
+        SW_HIP(e, hipStreamSynchronize(s));
+    }
+    if (ph->has_field && !(formats & (1u << SW_FIELD_F32))) {
+        dev_free(ph->field);
+        ph->field = nullptr;
+    }
+    return SW_OK;
+}
+
+static int phantom_create_common(sw_engine *e, const uint8_t *mask, const float *field, const uint64_t dims[3],
+                                 const float fov[3], uint32_t formats, sw_phantom **out, hipMemcpyKind kind) {
+    if (!e || !mask || !dims || !fov || !out) return fail(e, SW_E_INVALID, "null argument");
+    *out = nullptr;
+    SW_HIP(e, hipSetDevice(e->device));
+    std::unique_ptr<sw_phantom> ph(new sw_phantom);
+    ph->eng = e;
+    ph->device = e->device;
+    ph->n = dims[0] * dims[1] * dims[2];
+    if (ph->n == 0) return fail(e, SW_E_INVALID, "empty phantom");
+    for (int i = 0; i < 3; i++) { ph->dims[i] = dims[i]; ph->fov[i] = fov[i]; }
+    ph->has_field = field != nullptr;
+    if (formats == 0) formats = 1u << SW_FIELD_F32;
+    size_t need = ph->n;
+    if (ph->has_field) {
+        need += ph->n * 4;
+        if (formats & (1u << SW_FIELD_F16)) need += ph->n * 2;
+        if (formats & (1u << SW_FIELD_BF16)) need += ph->n * 2;
+    }
+    int rc = check_free_memory(e, need + (64u << 20));
+    if (rc) return rc;
+    hipStream_t s = e->stream;
+    SW_HIP(e, hipMalloc((void **)&ph->mask, ph->n));
+    SW_HIP(e, hipMemcpyAsync(ph->mask, mask, ph->n, kind, s));
+    if (ph->has_field) {
+        SW_HIP(e, hipMalloc((void **)&ph->field, ph->n * sizeof(float)));
+        SW_HIP(e, hipMemcpyAsync(ph->field, field, ph->n * sizeof(float), kind, s));
+    }
+    SW_HIP(e, hipStreamSynchronize(s));
+    rc = phantom_finish(e, ph.get(), formats);
+    if (rc) { sw_phantom_destroy(ph.release()); return rc; }
+    *out = ph.release();
+    return SW_OK;
+}
+
+SW_API int sw_phantom_create(sw_engine *e, const uint8_t *mask, const float *field_T, const uint64_t dims[3],
+                             const float fov[3], uint32_t formats, sw_phantom **out) {
+    return phantom_create_common(e, mask, field_T, dims, fov, formats, out, hipMemcpyHostToDevice);
+}
+
+SW_API int sw_phantom_create_device(sw_engine *e, const uint8_t *d_mask, const float *d_field_T,
+                                    const uint64_t dims[3], const float fov[3], uint32_t formats, sw_phantom **out) {
+    return phantom_create_common(e, d_mask, d_field_T, dims, fov, formats, out, hipMemcpyDeviceToDevice);
+}
+
+SW_API void sw_phantom_destroy(sw_phantom *ph) {
+    if (!ph) return;
+    (void)hipSetDevice(ph->device);
+    dev_free(ph->mask);
+    dev_free(ph->field);
+    dev_free(ph->field_f16);
+    dev_free(ph->field_bf16);
+    delete ph;
+}
+
+SW_API int sw_phantom_info(const sw_phantom *ph, uint32_t *n_labels, float *half_scale) {
+    if (!ph) return SW_E_INVALID;
+    if (n_labels) *n_labels = ph->n_labels;
+    if (half_scale) *half_scale = ph->half_scale;
+    return SW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sequence
+// ---------------------------------------------------------------------------
+SW_API int sw_sequence_create(sw_engine *e, const sw_params *p, const sw_seq *q, sw_sequence **out) {
+    if (!e || !p || !q || !out) return fail(e, SW_E_INVALID, "null argument");
+    *out = nullptr;
+    if (q->n_substrate == 0 || !q->diffusivity || !q->T1_ms || !q->T2_ms || !q->pXY)
+        return fail(e, SW_E_INVALID, "substrate tables are required");
+    if (q->n_rf == 0 || !q->RF_FA_deg || !q->RF_PH_deg || !q->RF_t)
+        return fail(e, SW_E_INVALID, "at least one RF pulse (the excitation) is required");
+    if ((q->n_te && !q->TE_t) || (q->n_dephasing && (!q->dephasing_t || !q->dephasing_deg)) ||
+        (q->n_gradient && (!q->gradient_t || !q->gradientX_mTm || !q->gradientY_mTm || !q->gradientZ_mTm)))
+        return fail(e, SW_E_INVALID, "table pointer missing for a non-empty table");
+    SW_HIP(e, hipSetDevice(e->device));
+    std::unique_ptr<sw_sequence> sq(new sw_sequence);
+    sq->eng = e;
+    sq->device = e->device;
+    const uint32_t ns = q->n_substrate;
+    sq->n_sub = ns; sq->n_rf = q->n_rf; sq->n_te = q->n_te; sq->n_deph = q->n_dephasing; sq->n_grad = q->n_gradient;
+    sq->RF_FA0 = q->RF_FA_deg[0];
+    sq->RF_PH0 = q->RF_PH_deg[0];
+    std::vector<float> T1s(ns), T2s(ns);
+    std::vector<double> T1d(ns), T2d(ns);
+    for (uint32_t i = 0; i < ns; i++) {
+        T1s[i] = (float)(q->T1_ms[i] * 1e-3);                    // kernels.cu:167-168
+        T2s[i] = (float)(q->T2_ms[i] * 1e-3);
+        T1d[i] = (double)q->T1_ms[i] * 1e-3;
+        T2d[i] = (double)q->T2_ms[i] * 1e-3;
+    }
+    std::vector<sw::RfPulse> rf(q->n_rf);
+    for (uint32_t k = 0; k < q->n_rf; k++) {
+        const float fa = q->RF_FA_deg[k], phs = q->RF_PH_deg[k];
+        sw::RfPulse &r = rf[k];
+        std::memset(&r, 0, sizeof r);
+        r.t = q->RF_t[k];
+        r.kind = rot_kind_f(phs);
+        r.kind_d = rot_kind_d((double)phs);
+        r.s = (float)sin(fa * kDeg2Rad);                         // kernels.cuh:205
+        r.c = (float)cos(fa * kDeg2Rad);
+        r.sp = sinf((float)(phs * kDeg2Rad));                    // kernels.cuh:184-185
+        r.cp = cosf((float)(phs * kDeg2Rad));
+        r.sd = sin((double)fa * kDeg2Rad);
+        r.cd = cos((double)fa * kDeg2Rad);
+        r.spd = sin((double)phs * kDeg2Rad);
+        r.cpd = cos((double)phs * kDeg2Rad);
+    }
+    hipStream_t s = e->stream;
+    SW_HIP(e, dev_upload(&sq->diffusivity, q->diffusivity, ns, s));
+    SW_HIP(e, dev_upload(&sq->pXY, q->pXY, (size_t)ns * ns, s));
+    SW_HIP(e, dev_upload(&sq->T1s, T1s.data(), ns, s));
+    SW_HIP(e, dev_upload(&sq->T2s, T2s.data(), ns, s));
+    SW_HIP(e, dev_upload(&sq->T1d, T1d.data(), ns, s));
+    SW_HIP(e, dev_upload(&sq->T2d, T2d.data(), ns, s));
+    SW_HIP(e, dev_upload(&sq->rf, rf.data(), rf.size(), s));
+    SW_HIP(e, dev_upload(&sq->TE_t, q->TE_t, q->n_te, s));
+    SW_HIP(e, dev_upload(&sq->deph_t, q->dephasing_t, q->n_dephasing, s));
+    SW_HIP(e, dev_upload(&sq->deph_deg, q->dephasing_deg, q->n_dephasing, s));
+    SW_HIP(e, dev_upload(&sq->grad_t, q->gradient_t, q->n_gradient, s));
+    SW_HIP(e, dev_upload(&sq->gx, q->gradientX_mTm, q->n_gradient, s));
+    SW_HIP(e, dev_upload(&sq->gy, q->gradientY_mTm, q->n_gradient, s));
+    SW_HIP(e, dev_upload(&sq->gz, q->gradientZ_mTm, q->n_gradient, s));
+    SW_HIP(e, hipStreamSynchronize(s));
+    int rc = ensure_exc(e, sq.get(), p, s);
+    if (rc) { sw_sequence_destroy(sq.release()); return rc; }
+    *out = sq.release();
+    return SW_OK;
+}
+
+SW_API void sw_sequence_destroy(sw_sequence *q) {
+    if (!q) return;
+    (void)hipSetDevice(q->device);
+    for (void *p : {(void *)q->diffusivity, (void *)q->pXY, (void *)q->T1s, (void *)q->T2s, (void *)q->T1d,
+                    (void *)q->T2d, (void *)q->rf, (void *)q->TE_t, (void *)q->deph_t, (void *)q->deph_deg,
+                    (void *)q->grad_t, (void *)q->gx, (void *)q->gy, (void *)q->gz, (void *)q->exc})
+        dev_free(p);
+    delete q;
+}
+
+// ---------------------------------------------------------------------------
+// the walk
+// ---------------------------------------------------------------------------
+namespace {
+
+// Visiting order: local spin indices sorted by the Morton index of the 8^3-voxel
+// brick holding their start position.  Neighbouring lanes then gather from
+// neighbouring voxels, and the XCD-aware block mapping of the walk kernel gives
+// each XCD (own L2) a compact region of the phantom.
+int build_order(sw_engine *e, const sw::WalkArgs &a, hipStream_t s) {
+    const uint32_t n = a.n_local;
+    const size_t tmp = sw::sort_pairs_temp_bytes(n);
+    const size_t need = 3 * (size_t)n * sizeof(uint32_t) + tmp + 256;
+    if (need > e->sort_bytes) {
+        SW_HIP(e, hipStreamSynchronize(s));
+        dev_free(e->d_sort);
+        e->d_sort = nullptr;
+        SW_HIP(e, hipMalloc(&e->d_sort, need));
+        e->sort_bytes = need;
+    }
+    if (n > e->order_cap) {
+        SW_HIP(e, hipStreamSynchronize(s));
+        dev_free(e->d_order);
+        e->d_order = nullptr;
+        SW_HIP(e, hipMalloc((void **)&e->d_order, n * sizeof(uint32_t)));
+        e->order_cap = n;
+    }
+    uint32_t *kin = (uint32_t *)e->d_sort, *kout = kin + n, *vin = kout + n;
+    void *t = (void *)(((uintptr_t)(vin + n) + 255) & ~(uintptr_t)255);
+    SW_HIP(e, sw::launch_brick_keys(a.XYZ0, a.xyz_scale, n, a.s2g, a.nx, a.ny, a.nz, kin, vin, s));
+    SW_HIP(e, sw::launch_sort_pairs(t, tmp, kin, kout, vin, e->d_order, n, s));
+    return SW_OK;
+}
+
+}  // namespace
+
+SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence *cq, const sw_params *p,
+                          const sw_walk_opts *o, const float *d_M0, const float *d_XYZ0, void *d_M1,
+                          float *d_XYZ1, uint8_t *d_T, void *stream, sw_walk_stats *stats) {
+    if (!e || !ph || !cq || !p || !o) return fail(e, SW_E_INVALID, "null argument");
+    sw_sequence *q = const_cast<sw_sequence *>(cq);
+    hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    SW_HIP(e, hipSetDevice(e->device));
+    // ---- consistency checks (the reference trusts its caller; an ABI may not)
+    for (int i = 0; i < 3; i++)
+        if (p->phantom_size[i] != ph->dims[i]) return fail(e, SW_E_INVALID, "phantom_size does not match phantom");
+    if (p->matrix_length != (int64_t)ph->n) return fail(e, SW_E_INVALID, "matrix_length does not match phantom");
+    if (p->n_substrate != q->n_sub) return fail(e, SW_E_INVALID, "n_substrate does not match sequence tables");
+    if (ph->n_labels > q->n_sub)                                  // monte_carlo.cu:113-118
+        return fail(e, SW_E_INVALID, "mask has " + std::to_string(ph->n_labels) + " labels but only " +
+                                         std::to_string(q->n_sub) + " substrates are configured");
+    if (p->timestep_us <= 0) return fail(e, SW_E_INVALID, "timestep_us must be positive");
+    if (!(p->fov[0] > 0 && p->fov[1] > 0 && p->fov[2] > 0)) return fail(e, SW_E_INVALID, "fov must be positive");
+    if ((uint64_t)o->spin_offset + o->n_local > p->n_spins)
+        return fail(e, SW_E_INVALID, "spin_offset + n_local exceeds n_spins");
+    if (o->n_local && (!d_M0 || !d_XYZ0 || !d_XYZ1 || (q->n_te && (!d_M1 || !d_T))))
+        return fail(e, SW_E_INVALID, "spin buffers are required");
+    if (o->precision < 0 || o->precision > 2) return fail(e, SW_E_INVALID, "unknown precision");
+    if (o->rng == SW_RNG_REPLAY && (!o->replay_normals || !o->replay_uniforms || o->replay_normal_stride < 3))
+        return fail(e, SW_E_INVALID, "replay mode needs normal and uniform streams");
+    if (o->rng != SW_RNG_REPLAY && o->rng != SW_RNG_PHILOX) return fail(e, SW_E_INVALID, "unknown rng");
+    int field = ph->has_field ? o->field_format : SW_FIELD_NONE;
+    if (field == SW_FIELD_F32 && !ph->field) return fail(e, SW_E_INVALID, "phantom has no float32 field copy");
+    if (field == SW_FIELD_F16 && !ph->field_f16) return fail(e, SW_E_INVALID, "phantom has no float16 field copy");
+    if (field == SW_FIELD_BF16 && !ph->field_bf16) return fail(e, SW_E_INVALID, "phantom has no bfloat16 field copy");
+    if (field < 0 || field > 3) return fail(e, SW_E_INVALID, "unknown field format");
+    if (o->precision == SW_PREC_F64 && field > SW_FIELD_F32)
+        return fail(e, SW_E_INVALID, "the FP64 path gathers the float32 field");
+    int rc = ensure_exc(e, q, p, s);
+    if (rc) return rc;
+
+    sw::WalkArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.mask = ph->mask;
+    a.field_f32 = ph->field;
+    a.field_h = field == SW_FIELD_F16 ? ph->field_f16 : ph->field_bf16;
+    a.t2d = sw_tesla2deg(p->B0, p->timestep_us);
+    a.h2d = ph->half_scale * a.t2d;
+    a.t2d_f64 = (double)p->B0 * p->timestep_us * 1e-6 * kGamma * kRad2Deg;
+    a.nx = (int64_t)ph->dims[0]; a.ny = (int64_t)ph->dims[1]; a.nz = (int64_t)ph->dims[2];
+    a.matrix_length = p->matrix_length;
+    for (int i = 0; i < 3; i++) {
+        a.fov[i] = p->fov[i];
+        a.s2g[i] = (double)p->phantom_size[i] / p->fov[i];       // kernels.cu:98
+    }
+    a.diffusivity = q->diffusivity; a.pXY = q->pXY; a.T1s = q->T1s; a.T2s = q->T2s; a.T1d = q->T1d; a.T2d = q->T2d;
+    a.n_sub = q->n_sub;
+    a.rf = q->rf; a.n_rf = q->n_rf;
+    a.TE_t = q->TE_t; a.n_te = q->n_te;
+    a.deph_t = q->deph_t; a.deph_deg = q->deph_deg; a.n_deph = q->n_deph;
+    a.grad_t = q->grad_t; a.gx = q->gx; a.gy = q->gy; a.gz = q->gz; a.n_grad = q->n_grad;
+    a.exc = q->exc;
+    a.exc_s = p->s; a.exc_c = p->c;
+    a.exc_sd = sin((double)q->RF_FA0 * kDeg2Rad);
+    a.exc_cd = cos((double)q->RF_FA0 * kDeg2Rad);
+    a.n_timepoints = p->n_timepoints;
+    a.n_dummy = (uint32_t)p->n_dummy_scan;
+    a.timestep_us = p->timestep_us;
+    a.n_spins_total = p->n_spins;
+    a.spin_offset = o->spin_offset;
+    a.n_local = o->n_local;
+    a.seed = p->seed;
+    a.max_iterations = p->max_iterations;
+    a.cross_fov = p->enCrossFOV ? 1 : 0;
+    a.record_traj = p->enRecordTrajectory ? 1 : 0;
+    a.xyz_scale = o->xyz_scale;
+    a.gradient_scale = o->gradient_scale;
+    a.traj_len = p->enRecordTrajectory ? (uint64_t)(p->n_dummy_scan + 1) * p->n_timepoints : 1;
+    a.M0 = d_M0; a.XYZ0 = d_XYZ0; a.M1 = d_M1; a.XYZ1 = d_XYZ1; a.T = d_T;
+    a.rn = o->replay_normals; a.rn_stride = o->replay_normal_stride;
+    a.ru = o->replay_uniforms; a.ru_stride = o->replay_uniform_stride;
+    a.counters = e->d_counters;
+    a.order = nullptr;
+    if (o->spatial_order && o->n_local > 1) {
+        rc = build_order(e, a, s);
+        if (rc) return rc;
+        a.order = e->d_order;
+    }
+    SW_HIP(e, sw::launch_zero_u64(e->d_counters, 4, s));
+    if (stats) SW_HIP(e, hipEventRecord(e->ev0, s));
+    SW_HIP(e, sw::launch_walk(a, o->rng == SW_RNG_REPLAY ? 1 : 0, field, o->precision, s));
+    if (stats) {
+        SW_HIP(e, hipEventRecord(e->ev1, s));
+        unsigned long long c[4];
+        SW_HIP(e, hipMemcpyAsync(c, e->d_counters, sizeof c, hipMemcpyDeviceToHost, s));
+        SW_HIP(e, hipStreamSynchronize(s));
+        float ms = 0;
+        SW_HIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        stats->n_lost = c[0]; stats->n_replay_exhausted = c[1]; stats->n_steps = c[2]; stats->n_attempts = c[3];
+        stats->kernel_ms = ms;
+        if (c[1]) return fail(e, SW_E_REPLAY, std::to_string(c[1]) + " replayed spins ran out of recorded draws");
+    }
+    return SW_OK;
+}
+
+SW_API int sw_walk_host(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, const sw_params *p,
+                        const sw_walk_opts *o, const float *M0, const float *XYZ0, void *M1, float *XYZ1,
+                        uint8_t *T, const float *rn, const float *ru, sw_walk_stats *stats) {
+    if (!e || !o || !p) return fail(e, SW_E_INVALID, "null argument");
+    SW_HIP(e, hipSetDevice(e->device));
+    const size_t n = o->n_local, nte = q ? q->n_te : 0;
+    const size_t traj = p->enRecordTrajectory ? (size_t)(p->n_dummy_scan + 1) * p->n_timepoints : 1;
+    const size_t m1b = (o->precision == SW_PREC_F64 ? 8 : 4) * 3 * n * nte;
+    hipStream_t s = e->stream;
+    float *dM0 = nullptr, *dX0 = nullptr, *dX1 = nullptr, *drn = nullptr, *dru = nullptr;
+    void *dM1 = nullptr;
+    uint8_t *dT = nullptr;
+    sw_walk_opts oo = *o;
+    sw_walk_stats st;
+    std::memset(&st, 0, sizeof st);
+    int rc = SW_OK;
+    size_t need = n * 24 + m1b + n * nte + traj * n * 12;
+    if (o->rng == SW_RNG_REPLAY) need += 4 * n * (o->replay_normal_stride + o->replay_uniform_stride);
+    rc = check_free_memory(e, need);
+    if (rc) return rc;
+    auto cleanup = [&]() {
+        for (void *x : {(void *)dM0, (void *)dX0, (void *)dX1, (void *)drn, (void *)dru, dM1, (void *)dT}) dev_free(x);
+    };
+#define SW_TRY(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t _e = (expr);                                                                   \
+        if (_e != hipSuccess) { cleanup(); return fail(e, SW_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); } \
+    } while (0)
+    SW_TRY(dev_upload(&dM0, M0, 3 * n, s));
+    SW_TRY(dev_upload(&dX0, XYZ0, 3 * n, s));
+    SW_TRY(dev_upload(&dX1, XYZ1, 3 * n * traj, s));
+    if (m1b) {
+        SW_TRY(hipMalloc(&dM1, m1b));
+        SW_TRY(hipMemcpyAsync(dM1, M1, m1b, hipMemcpyHostToDevice, s));
+    }
+    SW_TRY(dev_upload(&dT, T, n * nte, s));
+    if (o->rng == SW_RNG_REPLAY) {
+        SW_TRY(dev_upload(&drn, rn, n * o->replay_normal_stride, s));
+        SW_TRY(dev_upload(&dru, ru, n * o->replay_uniform_stride, s));
+        oo.replay_normals = drn;
+        oo.replay_uniforms = dru;
+        if (!drn || !dru) { cleanup(); return fail(e, SW_E_INVALID, "replay streams are empty"); }
+    }
+    rc = sw_walk_device(e, ph, q, p, &oo, dM0, dX0, dM1, dX1, dT, s, &st);
+    if (rc == SW_OK || rc == SW_E_REPLAY) {
+        if (m1b) SW_TRY(hipMemcpyAsync(M1, dM1, m1b, hipMemcpyDeviceToHost, s));
+        if (n * nte != 0) SW_TRY(hipMemcpyAsync(T, dT, n * nte, hipMemcpyDeviceToHost, s));
+        if (n) SW_TRY(hipMemcpyAsync(XYZ1, dX1, 3 * n * traj * sizeof(float), hipMemcpyDeviceToHost, s));
+        SW_TRY(hipStreamSynchronize(s));
+    }
+#undef SW_TRY
+    cleanup();
+    if (stats) *stats = st;
+    return rc;
+}
+
+SW_API int sw_echo_reduce_device(sw_engine *e, const void *d_M1, int m1_dtype, const uint8_t *d_T, uint32_t n_spins,
+                                 uint32_t n_te, uint32_t n_sub, double *d_out, void *stream) {
+    if (!e || !d_out || (n_spins && (!d_M1 || !d_T))) return fail(e, SW_E_INVALID, "null argument");
+    hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    SW_HIP(e, hipSetDevice(e->device));
+    const size_t need = sw::reduce_partial_len(n_spins, n_te, n_sub);
+    if (need > e->partial_cap) {
+        SW_HIP(e, hipStreamSynchronize(s));
+        dev_free(e->d_partial);
+        e->d_partial = nullptr;
+        SW_HIP(e, hipMalloc((void **)&e->d_partial, need * sizeof(double)));
+        e->partial_cap = need;
+    }
+    SW_HIP(e, sw::launch_echo_reduce(d_M1, m1_dtype == SW_DTYPE_F64, d_T, n_spins, n_te, n_sub, e->d_partial, d_out,
+                                     s));
+    return SW_OK;
+}
+
+// diagnostic: the production draws (not part of the reference interface)
+SW_API int sw_philox_draws_device(sw_engine *e, uint64_t seed, uint32_t spin0, uint32_t n_spins, uint32_t n_att,
+                                  float *d_out, void *stream) {
+    if (!e || !d_out) return fail(e, SW_E_INVALID, "null argument");
+    hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    SW_HIP(e, hipSetDevice(e->device));
+    SW_HIP(e, sw::launch_philox_draws(seed, spin0, n_spins, n_att, d_out, s));
+    return SW_OK;
+}
+
+SW_API int sw_stream_sync(sw_engine *e, void *stream) {
+    if (!e) return SW_E_INVALID;
+    SW_HIP(e, hipStreamSynchronize(stream ? (hipStream_t)stream : e->stream));
+    return SW_OK;
+}
+
+}  // extern "C"

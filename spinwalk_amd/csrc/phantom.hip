@@ -1,0 +1,107 @@
+// phantom.hip -- phantom preparation kernels: label count, field range and the
+// half-precision copies of the fieldmap.
+//
+// The reference keeps the fieldmap in float32 (src/sim/monte_carlo.cu:98-123,
+// 254).  The mixed-precision walk gathers a float16 or bfloat16 copy of
+// field_T / half_scale, where half_scale is a power of two chosen so that the
+// largest |field| lands near 2^14: scaling by a power of two is exact, and it
+// keeps the 1/r^2 and 1/r^3 tails of vessel fields out of the float16
+// subnormal range for as long as possible.
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include "walk_launch.hpp"
+
+namespace sw {
+namespace {
+
+__global__ void stats_kernel(const uint8_t *mask, const float *field, uint64_t n, uint32_t *stats) {
+    uint32_t mlab = 0, mabs = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        mlab = max(mlab, (uint32_t)mask[i]);
+        if (field) mabs = max(mabs, __float_as_uint(fabsf(field[i])));   // |f| bits order like values
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mlab = max(mlab, (uint32_t)__shfl_xor((int)mlab, o));
+        mabs = max(mabs, (uint32_t)__shfl_xor((int)mabs, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&stats[0], mlab);
+        atomicMax(&stats[1], mabs);
+    }
+}
+
+__global__ void to_half_kernel(const float *field, uint64_t n, float inv_scale, int bf16, uint16_t *out) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const float v = field[i] * inv_scale;
+        uint16_t h;
+        if (bf16) {
+            // round to nearest even on the upper 16 bits
+            const uint32_t b = __float_as_uint(v);
+            const uint32_t r = b + 0x7FFFu + ((b >> 16) & 1u);
+            h = (uint16_t)(r >> 16);
+        } else {
+            h = __half_as_ushort(__float2half_rn(v));
+        }
+        out[i] = h;
+    }
+}
+
+__global__ void brick_keys_kernel(const float *XYZ0, float xyz_scale, uint32_t n, double s0, double s1,
+                                  double s2, int64_t nx, int64_t ny, int64_t nz, uint32_t *keys,
+                                  uint32_t *vals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = (double)(XYZ0[3 * (uint64_t)i + 0] * xyz_scale) * s0;
+    const double y = (double)(XYZ0[3 * (uint64_t)i + 1] * xyz_scale) * s1;
+    const double z = (double)(XYZ0[3 * (uint64_t)i + 2] * xyz_scale) * s2;
+    // 8^3-voxel bricks in Morton order (10 bits per axis covers 8192 voxels)
+    auto clampb = [](double v, int64_t nmax) -> uint32_t {
+        int64_t b = (int64_t)v;
+        if (b < 0) b = 0;
+        if (b >= nmax) b = nmax - 1;
+        return (uint32_t)(b >> 3) & 1023u;
+    };
+    const uint32_t bx = clampb(x, nx), by = clampb(y, ny), bz = clampb(z, nz);
+    uint32_t k = 0;
+    for (int j = 0; j < 10; j++) {
+        k |= ((bz >> j) & 1u) << (3 * j);
+        k |= ((by >> j) & 1u) << (3 * j + 1);
+        k |= ((bx >> j) & 1u) << (3 * j + 2);
+    }
+    keys[i] = k;
+    vals[i] = i;
+}
+
+}  // namespace
+
+hipError_t launch_phantom_stats(const uint8_t *mask, const float *field, uint64_t n, uint32_t *stats,
+                                hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(stats_kernel, dim3((unsigned)blocks), dim3(256), 0, s, mask, field, n, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_field_to_half(const float *field, uint64_t n, float inv_scale, int bf16, uint16_t *out,
+                                hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(to_half_kernel, dim3((unsigned)blocks), dim3(256), 0, s, field, n, inv_scale, bf16,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t launch_brick_keys(const float *XYZ0, float xyz_scale, uint32_t n, const double *s2g, int64_t nx,
+                             int64_t ny, int64_t nz, uint32_t *keys, uint32_t *vals, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(brick_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, s, XYZ0, xyz_scale, n,
+                       s2g[0], s2g[1], s2g[2], nx, ny, nz, keys, vals);
+    return hipGetLastError();
+}
+
+}  // namespace sw

@@ -1,0 +1,60 @@
+// philox.hpp -- Philox4x32-10 counter-based generator and the per-attempt draw
+// bundle of the production walk.
+//
+// Philox4x32-10 follows Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3" (SC'11): 10 rounds of two 32x32->64 multiplies by
+// 0xD2511F53 / 0xCD9E8D57 with Weyl key bumps 0x9E3779B9 / 0xBB67AE85.
+//
+// One attempted step of a spin draws one Philox block keyed by the 64-bit seed
+// with counter (attempt_lo, attempt_hi, global_spin, 0).  The block gives the
+// three displacement normals (two Box-Muller pairs, the last sine dropped) and,
+// from the otherwise unused low bytes, the permeability uniform.  The stream of
+// a spin depends only on (seed, spin, attempt), never on the launch shape or on
+// how spins are split over launches or devices.  As in the reference, whose
+// per-spin generators are re-seeded with seed+spin for every scale
+// (src/sim/kernels.cu:77-88), the draws do not depend on the scale index.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define SW_HD __host__ __device__ __forceinline__
+#else
+#define SW_HD inline
+#endif
+
+namespace sw {
+
+struct U32x4 { uint32_t v[4]; };
+
+SW_HD U32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                          uint32_t k0, uint32_t k1) {
+    constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)M0 * c0;
+        const uint64_t p1 = (uint64_t)M1 * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += W0; k1 += W1;
+    }
+    U32x4 o;
+    o.v[0] = c0; o.v[1] = c1; o.v[2] = c2; o.v[3] = c3;
+    return o;
+}
+
+// 24-bit uniforms from the top bits of a word.
+SW_HD float u24_open(uint32_t r)   { return ((float)(r >> 8) + 0.5f) * 5.9604644775390625e-08f; }  // (0,1)
+SW_HD float u24_closed0(uint32_t r){ return (float)(r >> 8) * 5.9604644775390625e-08f; }           // [0,1)
+// Permeability uniform from the low bytes of three words: [0,1), 24 bits.
+SW_HD float u24_lowbytes(uint32_t a, uint32_t b, uint32_t c) {
+    const uint32_t x = (a & 0xFFu) | ((b & 0xFFu) << 8) | ((c & 0xFFu) << 16);
+    return (float)x * 5.9604644775390625e-08f;
+}
+
+}  // namespace sw

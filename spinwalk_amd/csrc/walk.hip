@@ -1,0 +1,477 @@
+// walk.hip -- the per-spin Monte-Carlo walk for gfx950 (MI355X).
+//
+// One lane walks one spin through the whole sequence, as the reference kernel
+// does (src/sim/kernels.cu:55-63 maps thread -> spin and runs sim()), but the
+// state is laid out for CDNA4: positions stay in FP64 registers (full-rate FP64
+// VALU on CDNA4, and FP64 is what the reference's trajectory arithmetic is,
+// kernels.cu:93,131-136), event times are held as "next event" registers so a
+// step costs four integer compares, per-event trigonometry is precomputed per
+// sequence, and the per-step work is: draw -> move -> index -> (gather on voxel
+// change) -> phase add.
+//
+// The step body follows src/sim/kernels.cu:72-233 expression for expression
+// (float vs double promotion, truncation to int64 in sub2ind, the order of
+// events within a step), so with replayed draws the trajectories, tissue labels
+// and phases are the reference CPU path's bit for bit.  This file is compiled
+// with -ffp-contract=off: a fused multiply-add would change roundings.
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <math.h>
+
+#include "philox.hpp"
+#include "walk_args.hpp"
+#include "walk_launch.hpp"
+
+namespace sw {
+namespace {
+
+constexpr double kDeg2Rad = 0.0174532925199433;   // simulation_parameters.cuh:24
+constexpr double kRad2Deg = 57.2957795130823;     // simulation_parameters.cuh:25
+constexpr double kGamma   = 267515315.;           // definitions.h:20
+constexpr uint32_t kNever = 0xFFFFFFFFu;
+
+enum : int { ST_OK = 0, ST_LOST = 1, ST_EXHAUSTED = 2 };
+
+// ---------------------------------------------------------------------------
+// Rotations (src/sim/kernels.cuh:82-152), generic over float/double.
+// ---------------------------------------------------------------------------
+template <class R>
+__device__ __forceinline__ void rot_apply(int kind, R s, R c, R sp, R cp, R m[3]) {
+    R a0 = m[0], a1 = m[1], a2 = m[2];
+    switch (kind) {
+    case ROT_X:    m[1] = c * a1 - s * a2;  m[2] = s * a1 + c * a2;  return;          // xrot(s,c)
+    case ROT_XNEG: m[1] = c * a1 - (-s) * a2; m[2] = (-s) * a1 + c * a2; return;      // xrot(-s,c)
+    case ROT_Y:    m[0] = c * a0 + s * a2;  m[2] = -s * a0 + c * a2; return;          // yrot(s,c)
+    case ROT_YNEG: m[0] = c * a0 + (-s) * a2; m[2] = -(-s) * a0 + c * a2; return;     // yrot(-s,c)
+    default: {
+        // zrot(-sp,cp) -> xrot(s,c) -> zrot(sp,cp)   (kernels.cuh:183-188)
+        const R nsp = -sp;
+        R b0 = cp * a0 - nsp * a1, b1 = nsp * a0 + cp * a1, b2 = a2;
+        R c0 = b0, c1 = c * b1 - s * b2, c2 = s * b1 + c * b2;
+        m[0] = cp * c0 - sp * c1;
+        m[1] = sp * c0 + cp * c1;
+        m[2] = c2;
+        return;
+    }
+    }
+}
+
+// dephase_relax (kernels.cu:45-52): zrot(phase deg) then relax(exp(-t/T1), exp(-t/T2)).
+__device__ __forceinline__ void dephase_relax(float m[3], float phase, float T1, float T2, float t) {
+    const float arg = (float)(phase * kDeg2Rad);            // float * double -> float argument
+    const float s = sinf(arg), c = cosf(arg);
+    const float a0 = m[0], a1 = m[1];
+    m[0] = c * a0 - s * a1;
+    m[1] = s * a0 + c * a1;
+    if (T1 >= 0 && T2 >= 0) {
+        // unqualified exp on a float argument resolves to double exp (kernels.cu:51)
+        const float e1 = (float)exp((double)(-t / T1));
+        const float e2 = (float)exp((double)(-t / T2));
+        m[0] = m[0] * e2;                                    // relax (kernels.cuh:214-219)
+        m[1] = m[1] * e2;
+        m[2] = (float)(1. + e1 * (m[2] - 1.));
+    }
+}
+__device__ __forceinline__ void dephase_relax(double m[3], double phase, double T1, double T2, double t) {
+    const double arg = phase * kDeg2Rad;
+    const double s = sin(arg), c = cos(arg);
+    const double a0 = m[0], a1 = m[1];
+    m[0] = c * a0 - s * a1;
+    m[1] = s * a0 + c * a1;
+    if (T1 >= 0 && T2 >= 0) {
+        const double e1 = exp(-t / T1), e2 = exp(-t / T2);
+        m[0] *= e2; m[1] *= e2; m[2] = 1. + e1 * (m[2] - 1.);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Phase accumulators.
+// ---------------------------------------------------------------------------
+struct PhaseF32 {          // reference: float accumulated_phase (kernels.cu:126,171)
+    using R = float;
+    float v = 0.f;
+    __device__ void add(float x) { v += x; }
+    __device__ void add_grad(double g) { v = (float)((double)v + g); }   // kernels.cu:185
+    __device__ float value() const { return v; }
+    __device__ void reset() { v = 0.f; }
+};
+struct PhaseKahan {        // FP32 with Kahan compensation
+    using R = float;
+    float v = 0.f, c = 0.f;
+    __device__ void add(float x) {
+        const float y = x - c;
+        const float t = v + y;
+        c = (t - v) - y;
+        v = t;
+    }
+    __device__ void add_grad(double g) { add((float)g); }
+    __device__ float value() const { return v - c; }
+    __device__ void reset() { v = 0.f; c = 0.f; }
+};
+struct PhaseF64 {
+    using R = double;
+    double v = 0.;
+    __device__ void add(double x) { v += x; }
+    __device__ void add_grad(double g) { v += g; }
+    __device__ double value() const { return v; }
+    __device__ void reset() { v = 0.; }
+};
+
+// ---------------------------------------------------------------------------
+// Field gathers.  The reference multiplies the fieldmap by the T->deg/step
+// factor in float once on the host (monte_carlo.cu:243-244); multiplying the
+// gathered value by the same float factor gives the same float.
+// ---------------------------------------------------------------------------
+struct FieldNone { template <class R> __device__ static R get(const WalkArgs &, int64_t) { return (R)0; } };
+struct FieldF32 {
+    template <class R> __device__ static R get(const WalkArgs &a, int64_t i);
+};
+template <> __device__ float FieldF32::get<float>(const WalkArgs &a, int64_t i) {
+    return a.field_f32[i] * a.t2d;
+}
+template <> __device__ double FieldF32::get<double>(const WalkArgs &a, int64_t i) {
+    return (double)a.field_f32[i] * a.t2d_f64;
+}
+struct FieldF16 {
+    template <class R> __device__ static R get(const WalkArgs &a, int64_t i) {
+        const __half h = __ushort_as_half(a.field_h[i]);
+        return (R)(__half2float(h) * a.h2d);
+    }
+};
+struct FieldBF16 {
+    template <class R> __device__ static R get(const WalkArgs &a, int64_t i) {
+        const float f = __uint_as_float((uint32_t)a.field_h[i] << 16);
+        return (R)(f * a.h2d);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Draw sources.
+// ---------------------------------------------------------------------------
+struct ReplayRng {          // the reference CPU path's recorded streams
+    const float *n, *u;
+    uint64_t nn, nu, in = 0, iu = 0;
+    __device__ ReplayRng(const WalkArgs &a, uint32_t li, uint32_t) {
+        n = a.rn + (uint64_t)li * a.rn_stride; nn = a.rn_stride;
+        u = a.ru + (uint64_t)li * a.ru_stride; nu = a.ru_stride;
+    }
+    __device__ bool normals(float &x, float &y, float &z) {
+        if (in + 3 > nn) return false;
+        x = n[in]; y = n[in + 1]; z = n[in + 2];
+        in += 3;
+        return true;
+    }
+    __device__ bool uniform(float &v) {
+        if (iu >= nu) return false;
+        v = u[iu++];
+        return true;
+    }
+};
+
+struct PhiloxRng {          // production: Philox4x32-10, one block per attempted step
+    uint32_t k0, k1, spin;
+    uint64_t attempt = 0;
+    float upend = 0.f;
+    __device__ PhiloxRng(const WalkArgs &a, uint32_t, uint32_t spin_global)
+        : k0((uint32_t)a.seed), k1((uint32_t)(a.seed >> 32)), spin(spin_global) {}
+    __device__ bool normals(float &x, float &y, float &z) {
+        const U32x4 r = philox4x32_10((uint32_t)attempt, (uint32_t)(attempt >> 32), spin, 0u, k0, k1);
+        ++attempt;
+        // Box-Muller on (r0,r1) and (r2,r3); v_log_f32 is log2, v_sin/v_cos take revolutions
+        const float u1 = u24_open(r.v[0]), v1 = u24_closed0(r.v[1]);
+        const float u2 = u24_open(r.v[2]), v2 = u24_closed0(r.v[3]);
+        const float rho1 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+        const float rho2 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u2));
+        x = rho1 * __builtin_amdgcn_cosf(v1);
+        y = rho1 * __builtin_amdgcn_sinf(v1);
+        z = rho2 * __builtin_amdgcn_cosf(v2);
+        upend = u24_lowbytes(r.v[0], r.v[1], r.v[2]);
+        return true;
+    }
+    __device__ bool uniform(float &v) { v = upend; return true; }
+};
+
+// ---------------------------------------------------------------------------
+// sub2ind on truncated scaled positions (kernels.cu:140, kernels.cuh:53-60).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t voxel_index(const WalkArgs &a, const double x[3]) {
+    const double vx = x[0] * a.s2g[0], vy = x[1] * a.s2g[1], vz = x[2] * a.s2g[2];
+    constexpr double kLim = 2147483648.0;
+    if (fabs(vx) < kLim && fabs(vy) < kLim && fabs(vz) < kLim) {
+        // v_cvt_i32_f64 truncates toward zero like the double->int64 conversion
+        const int64_t ix = (int32_t)vx, iy = (int32_t)vy, iz = (int32_t)vz;
+        return ix * a.nz * a.ny + iy * a.nz + iz;
+    }
+    const int64_t ix = (int64_t)vx, iy = (int64_t)vy, iz = (int64_t)vz;
+    return ix * a.nz * a.ny + iy * a.nz + iz;
+}
+
+__device__ __forceinline__ void move_axis(const WalkArgs &a, int i, float r, double diff,
+                                          const double xo[3], double xn[3]) {
+    const double rw = (double)r * diff;                          // kernels.cu:131
+    double v = xo[i] + rw;
+    if (v < 0)
+        v += (a.cross_fov ? a.fov[i] : 2 * fabs(rw));            // :133-134
+    else if (v >= a.fov[i])
+        v -= (a.cross_fov ? a.fov[i] : 2 * fabs(rw));            // :135-136
+    xn[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// One spin.  Returns ST_OK / ST_LOST / ST_EXHAUSTED and counts steps.
+// ---------------------------------------------------------------------------
+template <class Rng, class Field, class Phase>
+__device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint64_t &n_att) {
+    using R = typename Phase::R;
+    const uint32_t spin = a.spin_offset + li;
+    float *xyz1 = a.XYZ1 + 3 * (uint64_t)li * a.traj_len;
+    double xo[3], xn[3];
+    R m[3];
+    for (int i = 0; i < 3; i++) {                                  // kernels.cu:95-99
+        const float x0 = a.XYZ0[3 * (uint64_t)li + i] * a.xyz_scale;
+        xo[i] = xn[i] = (double)x0;
+        m[i] = (R)a.M0[3 * (uint64_t)li + i];
+        if (a.record_traj) xyz1[i] = x0;
+    }
+    int64_t ind = voxel_index(a, xo);                              // kernels.cu:102
+    if (ind < 0 || ind >= a.matrix_length) {                       // the reference reads out of bounds here
+        if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
+        return ST_LOST;
+    }
+    uint8_t ts_old = a.mask[ind];                                  // :103
+    double diff = a.diffusivity[ts_old];                           // :104
+    R field = 0, T1 = 0, T2 = 0;
+    uint32_t itr = 0;
+    Rng rng(a, li, spin);
+    Phase ph;
+    uint64_t slot = 0;
+    const int64_t nzny = a.nz * a.ny;
+    (void)nzny;
+
+    for (uint32_t d = 0; d <= a.n_dummy; ++d) {                    // :107
+        const bool last = (d == a.n_dummy);
+        {                                                          // :110-120 excitation
+            const Excitation e = a.exc[d];
+            if (sizeof(R) == 8)
+                rot_apply<R>(e.kind_d, (R)a.exc_sd, (R)a.exc_cd, (R)e.spd, (R)e.cpd, m);
+            else
+                rot_apply<R>(e.kind, (R)a.exc_s, (R)a.exc_c, (R)e.sp, (R)e.cp, m);
+        }
+        int64_t ind_old = a.matrix_length + 1;                     // :123-126
+        uint32_t tp = 0, old_tp = 0;
+        uint32_t rf_i = 1, te_i = 0, dp_i = 0, gr_i = 0;
+        uint32_t next_rf = (a.n_rf > 1) ? (uint32_t)a.rf[1].t : kNever;
+        uint32_t next_te = (last && a.n_te > 0) ? (uint32_t)a.TE_t[0] : kNever;
+        uint32_t next_dp = (a.n_deph > 0) ? (uint32_t)a.deph_t[0] : kNever;
+        uint32_t next_gr = (a.n_grad > 0) ? (uint32_t)a.grad_t[0] : kNever;
+        ph.reset();
+
+        while (tp < a.n_timepoints) {                              // :128
+            if (diff != 0.) {                                      // :130-137
+                float r0, r1, r2;
+                if (!rng.normals(r0, r1, r2)) return ST_EXHAUSTED;
+                move_axis(a, 0, r0, diff, xo, xn);
+                move_axis(a, 1, r1, diff, xo, xn);
+                move_axis(a, 2, r2, diff, xo, xn);
+            }
+            ++n_att;
+            ind = voxel_index(a, xn);                              // :140
+            if (ind >= a.matrix_length || ind < 0) {               // :141-147
+                if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
+                return ST_LOST;
+            }
+            if (ind != ind_old) {                                  // :150
+                const uint8_t ts = a.mask[ind];                    // :152
+                if (ts != ts_old) {                                // :153-163
+                    float u;
+                    if (!rng.uniform(u)) return ST_EXHAUSTED;
+                    if (u >= a.pXY[ts_old * a.n_sub + ts]) {
+                        if (itr++ > a.max_iterations) {
+                            if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
+                            return ST_LOST;
+                        }
+                        continue;
+                    }
+                    ts_old = ts;
+                }
+                ind_old = ind;                                     // :165-169
+                field = Field::template get<R>(a, ind);
+                if (sizeof(R) == 8) { T1 = (R)a.T1d[ts_old]; T2 = (R)a.T2d[ts_old]; }
+                else                { T1 = (R)a.T1s[ts_old]; T2 = (R)a.T2s[ts_old]; }
+                diff = a.diffusivity[ts_old];
+            }
+            ph.add(field);                                         // :171
+            itr = 0;
+
+            if (tp == next_dp) {                                   // :175-178
+                if (sizeof(R) == 8)
+                    ph.add((R)((double)spin * a.deph_deg[dp_i] / (double)a.n_spins_total));
+                else
+                    ph.add((R)((float)spin * a.deph_deg[dp_i] / (float)a.n_spins_total));
+                ++dp_i;
+                next_dp = (dp_i < a.n_deph) ? (uint32_t)a.deph_t[dp_i] : kNever;
+            }
+            if (tp == next_gr) {                                   // :181-187
+                const float Gx = a.gx[gr_i] * a.gradient_scale;
+                const float Gy = a.gy[gr_i] * a.gradient_scale;
+                const float Gz = a.gz[gr_i] * a.gradient_scale;
+                ph.add_grad((Gx * xn[0] + Gy * xn[1] + Gz * xn[2]) * 1e-3 * a.timestep_us * 1e-6 *
+                            kGamma * kRad2Deg);
+                ++gr_i;
+                next_gr = (gr_i < a.n_grad) ? (uint32_t)a.grad_t[gr_i] : kNever;
+            }
+            if (tp == next_rf) {                                   // :190-199
+                const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
+                dephase_relax(m, ph.value(), T1, T2, te);
+                const RfPulse p = a.rf[rf_i];
+                if (sizeof(R) == 8) rot_apply<R>(p.kind_d, (R)p.sd, (R)p.cd, (R)p.spd, (R)p.cpd, m);
+                else                rot_apply<R>(p.kind, (R)p.s, (R)p.c, (R)p.sp, (R)p.cp, m);
+                ph.reset();
+                old_tp = tp;
+                ++rf_i;
+                next_rf = (rf_i < a.n_rf) ? (uint32_t)a.rf[rf_i].t : kNever;
+            }
+            if (tp == next_te) {                                   // :202-215
+                const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
+                dephase_relax(m, ph.value(), T1, T2, te);
+                const uint64_t o = (uint64_t)li * a.n_te + te_i;
+                R *M1 = (R *)a.M1;
+                M1[3 * o + 0] = m[0]; M1[3 * o + 1] = m[1]; M1[3 * o + 2] = m[2];
+                a.T[o] = ts_old;
+                ph.reset();
+                old_tp = tp;
+                ++te_i;
+                next_te = (te_i < a.n_te) ? (uint32_t)a.TE_t[te_i] : kNever;
+            }
+            if (a.record_traj) {                                   // :218-221
+                float *dst = xyz1 + 3 * slot;
+                dst[0] = (float)xn[0]; dst[1] = (float)xn[1]; dst[2] = (float)xn[2];
+                ++slot;
+            }
+            xo[0] = xn[0]; xo[1] = xn[1]; xo[2] = xn[2];
+            ++tp;
+            ++n_acc;
+        }
+        const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);   // :226-231
+        dephase_relax(m, ph.value(), T1, T2, te);
+    }
+    if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
+    return ST_OK;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    // 64-lane butterfly; carries folded after each level
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t olo = __shfl_xor(lo, o), ohi = __shfl_xor(hi, o);
+        const uint32_t nlo = lo + olo;
+        hi = hi + ohi + (nlo < lo ? 1u : 0u);
+        lo = nlo;
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <class Rng, class Field, class Phase>
+__global__ __launch_bounds__(256) void walk_kernel(const WalkArgs a) {
+    // Blocks are dealt round-robin over the 8 XCDs; remap so that each XCD gets a
+    // contiguous range of the visiting order (spatially sorted spins share L2).
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t per = (nb + 7) / 8;
+    uint32_t lb = (b % 8) * per + b / 8;
+    if (nb % 8 != 0 || lb >= nb) lb = b;                       // exact remap only for multiples of 8
+    const uint32_t t = lb * blockDim.x + threadIdx.x;
+    uint64_t n_acc = 0, n_att = 0;
+    uint32_t lost = 0, exh = 0;
+    if (t < a.n_local) {
+        const uint32_t li = a.order ? a.order[t] : t;
+        const int st = walk_spin<Rng, Field, Phase>(a, li, n_acc, n_att);
+        lost = (st == ST_LOST);
+        exh = (st == ST_EXHAUSTED);
+    }
+    const uint64_t s_acc = wave_sum_u64(n_acc), s_att = wave_sum_u64(n_att);
+    const uint64_t s_lost = wave_sum_u64(lost), s_exh = wave_sum_u64(exh);
+    if ((threadIdx.x & 63) == 0 && a.counters) {
+        if (s_lost) atomicAdd(&a.counters[0], (unsigned long long)s_lost);
+        if (s_exh)  atomicAdd(&a.counters[1], (unsigned long long)s_exh);
+        atomicAdd(&a.counters[2], (unsigned long long)s_acc);
+        atomicAdd(&a.counters[3], (unsigned long long)s_att);
+    }
+}
+
+template <class Rng>
+hipError_t launch_rng(const WalkArgs &a, int field, int prec, hipStream_t s) {
+    const dim3 block(256), grid((a.n_local + 255) / 256);
+    if (grid.x == 0) return hipSuccess;
+#define SW_LAUNCH(F, P) hipLaunchKernelGGL((walk_kernel<Rng, F, P>), grid, block, 0, s, a)
+    if (prec == 2) {
+        if (field == 0) SW_LAUNCH(FieldNone, PhaseF64);
+        else if (field == 1) SW_LAUNCH(FieldF32, PhaseF64);
+        else return hipErrorInvalidValue;
+    } else if (prec == 1) {
+        if (field == 0) SW_LAUNCH(FieldNone, PhaseKahan);
+        else if (field == 1) SW_LAUNCH(FieldF32, PhaseKahan);
+        else if (field == 2) SW_LAUNCH(FieldF16, PhaseKahan);
+        else if (field == 3) SW_LAUNCH(FieldBF16, PhaseKahan);
+        else return hipErrorInvalidValue;
+    } else if (prec == 0) {
+        if (field == 0) SW_LAUNCH(FieldNone, PhaseF32);
+        else if (field == 1) SW_LAUNCH(FieldF32, PhaseF32);
+        else if (field == 2) SW_LAUNCH(FieldF16, PhaseF32);
+        else if (field == 3) SW_LAUNCH(FieldBF16, PhaseF32);
+        else return hipErrorInvalidValue;
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef SW_LAUNCH
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_walk(const WalkArgs &a, int rng, int field, int prec, hipStream_t s) {
+    if (rng == 1) return launch_rng<ReplayRng>(a, field, prec, s);
+    if (rng == 0) return launch_rng<PhiloxRng>(a, field, prec, s);
+    return hipErrorInvalidValue;
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The production draws, materialised (diagnostics and statistical tests).
+// out[(i*n_att + k)*4 + {0,1,2,3}] = {x, y, z, u} of attempt k of spin spin0+i.
+// ---------------------------------------------------------------------------
+__global__ void philox_draws_kernel(uint64_t seed, uint32_t spin0, uint32_t n_spins, uint32_t n_att,
+                                    float *out) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (uint64_t)n_spins * n_att) return;
+    const uint32_t i = (uint32_t)(g / n_att), k = (uint32_t)(g % n_att);
+    WalkArgs a{};
+    a.seed = seed;
+    PhiloxRng r(a, 0, spin0 + i);
+    r.attempt = k;
+    float x, y, z, u;
+    r.normals(x, y, z);
+    r.uniform(u);
+    out[4 * g + 0] = x; out[4 * g + 1] = y; out[4 * g + 2] = z; out[4 * g + 3] = u;
+}
+
+hipError_t launch_philox_draws(uint64_t seed, uint32_t spin0, uint32_t n_spins, uint32_t n_att,
+                               float *out, hipStream_t s) {
+    const uint64_t n = (uint64_t)n_spins * n_att;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(philox_draws_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       seed, spin0, n_spins, n_att, out);
+    return hipGetLastError();
+}
+
+}  // namespace sw

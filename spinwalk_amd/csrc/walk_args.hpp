@@ -1,0 +1,86 @@
+// walk_args.hpp -- argument block of the walk kernel and the small per-event
+// tables it reads.  Shared by the host side (engine.cpp, compiled by g++) and the
+// device side (walk.hip, compiled by hipcc); plain C++ only.
+#pragma once
+#include <stdint.h>
+
+namespace sw {
+
+// Which rotation xrot_withphase() reduces to for a given RF phase; the reference
+// tests the phase against 0, 180, 90 and -90/270 exactly (src/sim/kernels.cuh:162-181).
+enum RotKind : int32_t { ROT_X = 0, ROT_XNEG = 1, ROT_Y = 2, ROT_YNEG = 3, ROT_GENERAL = 4 };
+
+// One RF pulse after the excitation (RF index >= 1).  Trigonometry is formed on
+// the host exactly as the reference does it: FA through double sin/cos narrowed
+// to float (kernels.cuh:203-206), the phase through sinf/cosf of a float
+// argument (kernels.cuh:184-185).  *_d are the double versions for the FP64 path.
+struct RfPulse {
+    int32_t t;          // time index
+    int32_t kind;       // RotKind for the float path
+    int32_t kind_d;     // RotKind for the double path
+    int32_t pad;
+    float   s, c, sp, cp;
+    double  sd, cd, spd, cpd;
+};
+
+// Excitation of dummy scan d: RF_FA[0] with phase RF_PH[0] + d*lin + d(d+1)/2*quad
+// wrapped to [0, 360] (src/sim/kernels.cu:110-117).
+struct Excitation {
+    int32_t kind, kind_d;
+    float   sp, cp;
+    double  spd, cpd;
+};
+
+struct WalkArgs {
+    // ---- phantom -----------------------------------------------------------
+    const uint8_t  *mask;
+    const float    *field_f32;    // tesla per 1 T (NULL: no field)
+    const uint16_t *field_h;      // fp16/bf16 bits of field_T / half_scale
+    float           t2d;          // float T -> deg/step factor (monte_carlo.cu:241)
+    float           h2d;          // half -> deg/step factor = half_scale * t2d
+    double          t2d_f64;      // double T -> deg/step factor for the FP64 path
+    int64_t         nx, ny, nz, matrix_length;
+    double          fov[3];
+    double          s2g[3];       // phantom_size / fov (kernels.cu:98)
+
+    // ---- sequence tables (device) -------------------------------------------
+    const double   *diffusivity;  // [n_sub] metres per axis per step
+    const float    *pXY;          // [n_sub^2]
+    const float    *T1s, *T2s;    // [n_sub] seconds, (float)(ms * 1e-3) (kernels.cu:167-168)
+    const double   *T1d, *T2d;    // [n_sub] seconds in double
+    uint32_t        n_sub;
+    const RfPulse  *rf;           // [n_rf]; entry 0 unused (excitation)
+    uint32_t        n_rf;
+    const int32_t  *TE_t;         uint32_t n_te;
+    const int32_t  *deph_t;       const float *deph_deg; uint32_t n_deph;
+    const int32_t  *grad_t;       const float *gx, *gy, *gz; uint32_t n_grad;
+    const Excitation *exc;        // [n_dummy + 1]
+    float           exc_s, exc_c; // param.s, param.c
+    double          exc_sd, exc_cd;
+
+    // ---- scan ----------------------------------------------------------------
+    uint32_t        n_timepoints, n_dummy;
+    int32_t         timestep_us;
+    uint32_t        n_spins_total;
+    uint32_t        spin_offset, n_local;
+    uint64_t        seed, max_iterations;
+    int32_t         cross_fov, record_traj;
+    float           xyz_scale, gradient_scale;
+    uint64_t        traj_len;     // XYZ1 slots per spin
+
+    // ---- spins ---------------------------------------------------------------
+    const float    *M0, *XYZ0;
+    void           *M1;
+    float          *XYZ1;
+    uint8_t        *T;
+    const uint32_t *order;        // visiting order (local indices) or NULL
+
+    // ---- replay --------------------------------------------------------------
+    const float    *rn; uint64_t rn_stride;
+    const float    *ru; uint64_t ru_stride;
+
+    // ---- counters: [0] lost, [1] replay exhausted, [2] accepted steps, [3] attempts
+    unsigned long long *counters;
+};
+
+}  // namespace sw

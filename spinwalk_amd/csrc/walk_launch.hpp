@@ -1,0 +1,46 @@
+// walk_launch.hpp -- host-callable launchers of the HIP kernels (defined in the
+// .hip files, called from engine.cpp).
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include "walk_args.hpp"
+
+namespace sw {
+
+// The walk.  rng: 0 Philox, 1 replay.  field: 0 none, 1 f32, 2 f16, 3 bf16.
+// prec: 0 f32, 1 f32+Kahan, 2 f64.
+hipError_t launch_walk(const WalkArgs &a, int rng, int field, int prec, hipStream_t s);
+
+// Production draws of spins [spin0, spin0+n_spins), attempts [0, n_att).
+hipError_t launch_philox_draws(uint64_t seed, uint32_t spin0, uint32_t n_spins, uint32_t n_att,
+                               float *out, hipStream_t s);
+
+// Phantom preparation: max label and max |field| (into stats[0], stats[1] as
+// uint32 bit patterns; stats must be zeroed by the caller), then half copies.
+hipError_t launch_phantom_stats(const uint8_t *mask, const float *field, uint64_t n,
+                                uint32_t *stats, hipStream_t s);
+hipError_t launch_field_to_half(const float *field, uint64_t n, float inv_scale, int bf16,
+                                uint16_t *out, hipStream_t s);
+
+// FP64 echo reduction into out[(te*n_sub+label)*3 + {0,1,2}]; partial is scratch
+// of reduce_partial_len(n_spins, n_te, n_sub) doubles.
+uint64_t reduce_partial_len(uint32_t n_spins, uint32_t n_te, uint32_t n_sub);
+hipError_t launch_echo_reduce(const void *M1, int f64, const uint8_t *T, uint32_t n_spins,
+                              uint32_t n_te, uint32_t n_sub, double *partial, double *out,
+                              hipStream_t s);
+
+// Clears n counters with a fill kernel (no memset node, safe under graph capture).
+hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
+
+// Radix sort of (key, value) pairs (hipCUB); temp of sort_pairs_temp_bytes(n).
+size_t sort_pairs_temp_bytes(uint32_t n);
+hipError_t launch_sort_pairs(void *temp, size_t temp_bytes, const uint32_t *kin, uint32_t *kout,
+                             const uint32_t *vin, uint32_t *vout, uint32_t n, hipStream_t s);
+
+// Spatial visiting order: keys = brick id of each spin's start voxel.
+hipError_t launch_brick_keys(const float *XYZ0, float xyz_scale, uint32_t n, const double *s2g,
+                             int64_t nx, int64_t ny, int64_t nz, uint32_t *keys, uint32_t *vals,
+                             hipStream_t s);
+
+}  // namespace sw
