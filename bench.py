@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Throughput of the SpinWalk `sim` hot path on one MI355X, in spin-steps/s.
+
+Workload (default ``c2``): a 500^3 voxel, 500 um phantom of randomly oriented
+cylinders (r 5 um, 4 %) and spheres (r 3 um, 2 %) generated on the device, 1e6
+spins, spin echo 90/180 at TE 40 ms with dt 20 us (2001 steps per spin),
+permeability 0.5 between the three substrates.  One "step" of this benchmark is
+one complete simulation of that workload (every spin walks every time step)
+followed by the FP64 echo reduction; inputs are resident in HBM when the timed
+region starts.  The value is accepted spin-steps per second.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 is refused: this engine runs on one
+GPU (spins shard trivially across GPUs; that path is not built here).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+PRECISIONS = {"f32": 0, "kahan": 1, "f64": 2}
+FIELDS = {"f32": 1, "f16": 2, "bf16": 3}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(case, ph, budget_s, threads):
+    """The oracle (a C++ restatement of the reference CPU path, with its
+    per-spin mt19937 discard) on the first spins of the same workload, spread
+    over `threads` host threads, until `budget_s` seconds of wall time are used."""
+    from oracle import oracle as orc
+    mask, field = ph.download(field=True)
+    d = orc.from_engine_case(case, 0, mask=mask, field_T=field)
+    n = case.n_spins
+    chunk = max(1000, min(n, 2000 * threads))
+    done, steps, wall, disc = 0, 0, 0.0, 0.0
+    while done < n and wall < budget_s:
+        b, e = done, min(n, done + chunk)
+        t0 = time.perf_counter()
+        r = orc.sim(d, spin_begin=b, spin_end=e, n_threads=threads, timing=True)
+        wall += time.perf_counter() - t0
+        disc += r["timing"]["discard_s"]
+        lost = int(r["lost"][b:e].sum())
+        steps += (e - b - lost) * case.steps_per_spin()
+        done = e
+    rate = steps / wall
+    walk_only = steps / max(wall - disc, 1e-9)
+    return dict(value=rate, unit="spin-steps/s", cores=threads, kind="port",
+                sample=f"spins [0, {done}) of {n} (all {case.steps_per_spin()} steps each, "
+                       f"{steps:.3g} spin-steps, {wall:.1f} s wall); the rate includes the reference's per-spin "
+                       f"mt19937 discard(seed+spin), which grows with the spin index, so the full run would be "
+                       f"slower; walk-only rate {walk_only:.4g} spin-steps/s",
+                walk_only_value=walk_only)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--precision", default="f32", choices=sorted(PRECISIONS))
+    ap.add_argument("--field", default="f32", choices=sorted(FIELDS))
+    ap.add_argument("--order", type=int, default=1, help="spatial visiting order (1) or spin order (0)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="wall-time budget of the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the timed region, run one device copy of this size (profiler calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this engine runs on one GPU; --gpus must be 1", file=sys.stderr)
+        return 2
+
+    log("[bench] importing torch")
+    import torch
+    from spinwalk_amd import engine as E
+    from spinwalk_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    eng = E.Engine(0)
+    wl = W.WORKLOADS[args.workload]()
+    fmt = FIELDS[args.field]
+    prec = PRECISIONS[args.precision]
+    log(f"[bench] workload {wl['name']}: generating phantom on the device")
+    t0 = time.perf_counter()
+    ph, case, vf = W.build(eng, wl, formats=(E.SW_FIELD_F32, fmt))
+    log(f"[bench] phantom ready in {time.perf_counter() - t0:.1f} s, analytic volume fractions {vf}")
+    p = case.params
+    seq = eng.sequence(p, case.tables)
+    n, n_te = case.n_spins, case.n_te
+    M0 = torch.from_numpy(case.M0).to(dev)
+    XYZ0 = torch.from_numpy(case.XYZ0).to(dev)
+    M1 = torch.zeros((n, n_te, 3), dtype=torch.float64 if prec == 2 else torch.float32, device=dev)
+    T = torch.zeros((n, n_te), dtype=torch.uint8, device=dev)
+    XYZ1 = torch.zeros((n, case.traj_len, 3), dtype=torch.float32, device=dev)
+    sums = torch.zeros((n_te, p.n_substrate, 3), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def one_step():
+        st = eng.walk_device(ph, seq, p, M0, XYZ0, M1, XYZ1, T, rng=E.SW_RNG_PHILOX, precision=prec,
+                             field_format=fmt, spatial_order=args.order, stream=stream, stats=True)
+        eng.echo_reduce(M1, T, p.n_substrate, sums, stream=stream)
+        return st
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    kms, steps, lost = [], 0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = one_step()
+        kms.append(st["kernel_ms"])
+        steps += st["n_steps"]
+        lost += st["n_lost"]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if args.copy_mib > 0:
+        a = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+    value = steps / wall
+    bpe = 5 if fmt == E.SW_FIELD_F32 else 3           # u8 mask + field bytes per spin-step
+    if prec == 2:
+        bpe = 5
+    steps_per_launch = steps / max(1, args.steps)
+    kernel_s = float(np.mean(kms)) * 1e-3
+    achieved = steps_per_launch * bpe / kernel_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.precision}_{args.field}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    s = sums.cpu().numpy()
+    for te in range(n_te):
+        for lab in range(p.n_substrate):
+            if s[te, lab, 2] > 0:
+                mag = float(np.hypot(s[te, lab, 0], s[te, lab, 1]) / s[te, lab, 2])
+                log(f"[bench] echo {te} label {lab}: |<Mxy>| = {mag:.6f} over {int(s[te, lab, 2])} spins")
+    log(f"[bench] walk kernel {np.mean(kms):.3f} ms (min {np.min(kms):.3f}), {steps_per_launch:.4g} accepted "
+        f"spin-steps/launch, lost {lost}")
+    out = {
+        "metric": "spin-steps/s",
+        "value": value,
+        "unit": "spin-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {0: "f32", 1: "f32-kahan", 2: "f64"}[prec] + ("" if fmt == 1 else f"+{args.field}-field"),
+        "data": "synthetic",
+        "config": {"workload": wl["name"], "phantom": "500^3 voxels, 500 um, random cylinders r5 4% + spheres r3 2%"
+                   if args.workload == "c2" else str(wl["phantom"]["dims"]), "spins": n,
+                   "steps_per_spin": case.steps_per_spin(), "sequence": "SE 90/180 TE 40 ms dt 20 us"
+                   if args.workload == "c2" else "GRE", "precision": args.precision, "field": args.field,
+                   "spatial_order": args.order, "rng": "philox4x32-10"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_spin_step": bpe, "kernel_ms": float(np.mean(kms))},
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"[bench] CPU baseline: oracle on {threads} threads, {args.cpu_seconds} s budget")
+        out["cpu_baseline"] = cpu_baseline(case, ph, args.cpu_seconds, threads)
+    print(json.dumps(out), flush=True)
+    ph.close()
+    seq.close()
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -158,6 +158,34 @@ SW_API void sw_phantom_destroy(sw_phantom *ph);
 /* max(mask)+1 (src/sim/monte_carlo.cu:113) and the half-precision scale in use. */
 SW_API int  sw_phantom_info(const sw_phantom *ph, uint32_t *n_labels, float *half_scale);
 
+/* Copies a phantom back to the host (mask uint8 and field_T float32, either may
+ * be NULL; field_T needs a float32 copy). */
+SW_API int  sw_phantom_download(const sw_phantom *ph, uint8_t *mask, float *field_T);
+
+/* ---- phantom generation on the device ---------------------------------------
+ * Replaces the reference's host generators cylinder::run and sphere::run
+ * (src/phantom/phantom_cylinder.cpp:85-301, src/phantom/phantom_sphere.cpp:79-198):
+ * seeded non-overlapping cylinders (any axis) and spheres, labelled voxels and
+ * the closed-form susceptibility field summed per voxel, B0 along z.  Specs are
+ * applied in order (later labels win where shapes overlap).  achieved_vf
+ * (nullable) receives the analytic volume fraction placed per spec, percent. */
+#define SW_SHAPE_CYLINDER   0
+#define SW_SHAPE_SPHERE     1
+typedef struct sw_shape_spec {
+    int32_t  kind;                /* SW_SHAPE_*                                       */
+    uint32_t label;               /* mask value inside the shape (1..255)             */
+    float    radius_um;
+    float    volume_fraction;     /* percent of the box                               */
+    int32_t  random_orientation;  /* cylinders: isotropic axes                        */
+    float    orientation_deg;     /* cylinders: axis tilt from B0 otherwise           */
+    float    dchi, Y;             /* field scale 2pi/4pi * (1-Y) * dchi              */
+    uint32_t seed;
+    float    trunc_radii;         /* field truncation, in radii (0: cylinders none, spheres 10) */
+} sw_shape_spec;
+SW_API int  sw_phantom_generate(sw_engine *eng, const uint64_t dims[3], const float fov_um[3],
+                                const sw_shape_spec *specs, uint32_t n_specs, uint32_t formats,
+                                sw_phantom **out, double *achieved_vf);
+
 /* ---- sequence tables (parameters_dvec::copy_from_host) ---------------------- */
 SW_API int  sw_sequence_create(sw_engine *eng, const sw_params *p, const sw_seq *q, sw_sequence **out);
 SW_API void sw_sequence_destroy(sw_sequence *seq);

@@ -292,3 +292,51 @@ def sim(case, spin_begin=0, spin_end=None, n_threads=8, f64=False, timing=False)
     del keep
     return dict(M1=M1, XYZ1=XYZ1, T=T, lost=lost, n_lost=int(nlost),
                 timing=dict(discard_s=float(tm[0]), total_s=float(tm[1])))
+
+
+def from_engine_case(case, scale_index=0, mask=None, field_T=None):
+    """Oracle inputs for one scale of an engine ``Case`` (monte_carlo.cu:241-320),
+    every host transform recomputed by the oracle's own restatement.  ``mask`` and
+    ``field_T`` override the case's arrays (a device-generated phantom)."""
+    p = case.params
+    raw = case.raw
+    dt = int(np.float32(raw["TIME_STEP"]))
+    TR = int(np.float32(raw["TR"]))
+    tabs = {}
+    for key, name in (("TE", "TE_t"), ("RF_T", "RF_t"), ("DEPHASING_T", "dephasing_t"), ("GRADIENT_T", "gradient_t")):
+        v = raw.get(key, [])
+        tabs[name] = timing_scale([int(np.float32(x)) for x in v], dt) if len(v) else np.zeros(0, np.int32)
+    diff = np.asarray([float(np.float32(x)) for x in raw["DIFFUSIVITY"]], np.float64)
+    prep = prepare(np.float32(raw["RF_FA"][0]), TR, dt, np.float32(raw["T1"][0]), diff, int(raw.get("DUMMY_SCAN", 0)))
+    scale = np.float32(case.scales[scale_index])
+    st = case.scale_type
+    fov = np.asarray(case.fov, np.float32)
+    if st == 0:
+        fov_s = (scale * fov).astype(np.float32)                       # monte_carlo.cu:280
+        xyz0 = (case.XYZ0 * scale).astype(np.float32)                  # monte_carlo.cu:278
+    else:
+        fov_s, xyz0 = fov, case.XYZ0
+    mask = case.mask if mask is None else mask
+    field_T = case.field_T if field_T is None else field_T
+    lin = np.float32(raw.get("LINEAR_PHASE_CYCLING", 0.0))
+    d = dict(
+        fov=[float(x) for x in fov_s], phantom_size=list(mask.shape),
+        c=prep["c"], s=prep["s"],
+        linear_phase_cycling=float(lin * scale) if st == 2 else float(lin),
+        quadratic_phase_cycling=float(np.float32(raw.get("QUADRATIC_PHASE_CYCLING", 0.0))),
+        timestep_us=dt, TR_us=TR, n_dummy_scan=prep["n_dummy_scan"], n_spins=int(p.n_spins),
+        n_timepoints=prep["n_timepoints"], seed=int(p.seed), max_iterations=int(p.max_iterations),
+        cross_fov=int(p.enCrossFOV), record_trajectory=int(p.enRecordTrajectory),
+        diffusivity=prep["diffusivity"], T1_ms=np.asarray(raw["T1"], np.float32),
+        T2_ms=np.asarray(raw["T2"], np.float32), pXY=np.asarray(raw["P_XY"], np.float32).ravel(),
+        RF_FA_deg=np.asarray(raw["RF_FA"], np.float32), RF_PH_deg=np.asarray(raw["RF_PH"], np.float32),
+        RF_t=tabs["RF_t"], TE_t=tabs["TE_t"],
+        dephasing_deg=np.asarray(raw.get("DEPHASING", []), np.float32), dephasing_t=tabs["dephasing_t"],
+        gx=np.asarray(raw.get("GRADIENT_X", []), np.float32), gy=np.asarray(raw.get("GRADIENT_Y", []), np.float32),
+        gz=np.asarray(raw.get("GRADIENT_Z", []), np.float32), gradient_t=tabs["gradient_t"],
+        gradient_scale=float(scale) if st == 1 else 1.0,
+        mask=mask, M0=case.M0, XYZ0_scaled=xyz0, B0=float(np.float32(raw.get("B0", 9.4))),
+        field_T=field_T, prep=prep,
+    )
+    d["field_deg"] = None if field_T is None else scale_field(field_T, tesla2deg(d["B0"], dt))
+    return d

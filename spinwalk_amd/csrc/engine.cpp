@@ -21,6 +21,7 @@
 #include "../../include/spinwalk_engine.h"
 #include "walk_args.hpp"
 #include "walk_launch.hpp"
+#include "phantom_gen.hpp"
 
 #define SW_VERSION_STRING "spinwalk-mi355x 0.1.0 (drop-in for SpinWalk 1.21.0 sim)"
 
@@ -417,6 +418,105 @@ SW_API int sw_phantom_info(const sw_phantom *ph, uint32_t *n_labels, float *half
     if (!ph) return SW_E_INVALID;
     if (n_labels) *n_labels = ph->n_labels;
     if (half_scale) *half_scale = ph->half_scale;
+    return SW_OK;
+}
+
+SW_API int sw_phantom_download(const sw_phantom *ph, uint8_t *mask, float *field_T) {
+    if (!ph) return SW_E_INVALID;
+    sw_engine *e = ph->eng;
+    SW_HIP(e, hipSetDevice(ph->device));
+    if (mask) SW_HIP(e, hipMemcpy(mask, ph->mask, ph->n, hipMemcpyDeviceToHost));
+    if (field_T) {
+        if (!ph->field) return fail(e, SW_E_INVALID, "phantom has no float32 field copy");
+        SW_HIP(e, hipMemcpy(field_T, ph->field, ph->n * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return SW_OK;
+}
+
+SW_API int sw_phantom_generate(sw_engine *e, const uint64_t dims[3], const float fov_um[3], const sw_shape_spec *specs,
+                               uint32_t n_specs, uint32_t formats, sw_phantom **out, double *achieved_vf) {
+    if (!e || !dims || !fov_um || (n_specs && !specs) || !out) return fail(e, SW_E_INVALID, "null argument");
+    *out = nullptr;
+    SW_HIP(e, hipSetDevice(e->device));
+    std::vector<sw::GenCylinder> cyl;
+    std::vector<sw::GenSphere> sph;
+    for (uint32_t i = 0; i < n_specs; i++) {
+        sw::ShapeSpec ss;
+        ss.kind = specs[i].kind;
+        if (specs[i].label == 0 || specs[i].label > 255) return fail(e, SW_E_INVALID, "shape label must be 1..255");
+        if (!(specs[i].radius_um > 0)) return fail(e, SW_E_INVALID, "shape radius must be positive");
+        ss.label = (uint8_t)specs[i].label;
+        ss.radius_um = specs[i].radius_um;
+        ss.volume_fraction = specs[i].volume_fraction;
+        ss.random_orientation = specs[i].random_orientation;
+        ss.orientation_deg = specs[i].orientation_deg;
+        ss.dchi = specs[i].dchi;
+        ss.Y = specs[i].Y;
+        ss.seed = specs[i].seed;
+        ss.trunc_radii = specs[i].trunc_radii;
+        double vf = 0;
+        if (ss.kind == SW_SHAPE_CYLINDER) vf = sw::sample_cylinders(ss, fov_um, cyl);
+        else if (ss.kind == SW_SHAPE_SPHERE) vf = sw::sample_spheres(ss, fov_um, cyl, sph);
+        else return fail(e, SW_E_INVALID, "unknown shape kind");
+        if (achieved_vf) achieved_vf[i] = vf;
+    }
+    // cell list of spheres; a cell is at least as large as every truncation radius
+    float cell = 1.f;
+    for (auto &s : sph) cell = std::max(cell, std::sqrt(s.trunc2));
+    uint32_t nc[3];
+    for (int i = 0; i < 3; i++) nc[i] = std::max(1u, (uint32_t)std::ceil(fov_um[i] / cell));
+    const size_t ncells = (size_t)nc[0] * nc[1] * nc[2];
+    std::vector<uint32_t> start(ncells + 1, 0), cid(sph.size());
+    for (size_t k = 0; k < sph.size(); k++) {
+        uint32_t c3[3];
+        for (int i = 0; i < 3; i++) c3[i] = std::min(nc[i] - 1, (uint32_t)std::max(0.f, sph[k].c[i] / cell));
+        cid[k] = (c3[0] * nc[1] + c3[1]) * nc[2] + c3[2];
+        start[cid[k] + 1]++;
+    }
+    for (size_t c = 0; c < ncells; c++) start[c + 1] += start[c];
+    std::vector<sw::GenSphere> sorted(sph.size());
+    {
+        std::vector<uint32_t> fillp(start.begin(), start.end() - 1);
+        for (size_t k = 0; k < sph.size(); k++) sorted[fillp[cid[k]]++] = sph[k];
+    }
+    std::unique_ptr<sw_phantom> ph(new sw_phantom);
+    ph->eng = e;
+    ph->device = e->device;
+    ph->n = dims[0] * dims[1] * dims[2];
+    if (ph->n == 0) return fail(e, SW_E_INVALID, "empty phantom");
+    for (int i = 0; i < 3; i++) { ph->dims[i] = dims[i]; ph->fov[i] = fov_um[i] * 1e-6f; }
+    ph->has_field = true;
+    if (formats == 0) formats = 1u << SW_FIELD_F32;
+    int rc = check_free_memory(e, ph->n * 9 + (64u << 20));
+    if (rc) return rc;
+    hipStream_t s = e->stream;
+    sw::GenCylinder *dcyl = nullptr;
+    sw::GenSphere *dsph = nullptr;
+    uint32_t *dstart = nullptr;
+    SW_HIP(e, hipMalloc((void **)&ph->mask, ph->n));
+    SW_HIP(e, hipMalloc((void **)&ph->field, ph->n * sizeof(float)));
+    SW_HIP(e, dev_upload(&dcyl, cyl.data(), cyl.size(), s));
+    SW_HIP(e, dev_upload(&dsph, sorted.data(), sorted.size(), s));
+    SW_HIP(e, dev_upload(&dstart, start.data(), start.size(), s));
+    sw::GenArgs g;
+    std::memset(&g, 0, sizeof g);
+    g.n = ph->n;
+    g.nx = (uint32_t)dims[0]; g.ny = (uint32_t)dims[1]; g.nz = (uint32_t)dims[2];
+    for (int i = 0; i < 3; i++) g.vs[i] = fov_um[i] / (float)dims[i];
+    g.cyl = dcyl; g.n_cyl = (uint32_t)cyl.size();
+    g.sph = dsph; g.n_sph = (uint32_t)sorted.size();
+    g.cell_start = dstart;
+    for (int i = 0; i < 3; i++) g.nc[i] = nc[i];
+    g.cell = cell;
+    g.background = 0;
+    g.mask = ph->mask;
+    g.field = ph->field;
+    SW_HIP(e, sw::launch_phantom_gen(g, s));
+    SW_HIP(e, hipStreamSynchronize(s));
+    dev_free(dcyl); dev_free(dsph); dev_free(dstart);
+    rc = phantom_finish(e, ph.get(), formats);
+    if (rc) { sw_phantom_destroy(ph.release()); return rc; }
+    *out = ph.release();
     return SW_OK;
 }
 

@@ -376,10 +376,9 @@ template <class Rng, class Field, class Phase>
 __global__ __launch_bounds__(256) void walk_kernel(const WalkArgs a) {
     // Blocks are dealt round-robin over the 8 XCDs; remap so that each XCD gets a
     // contiguous range of the visiting order (spatially sorted spins share L2).
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t per = (nb + 7) / 8;
-    uint32_t lb = (b % 8) * per + b / 8;
-    if (nb % 8 != 0 || lb >= nb) lb = b;                       // exact remap only for multiples of 8
+    // The grid is a multiple of 8 blocks, so the remap is a permutation.
+    const uint32_t per = gridDim.x / 8, b = blockIdx.x;
+    const uint32_t lb = (b % 8) * per + b / 8;
     const uint32_t t = lb * blockDim.x + threadIdx.x;
     uint64_t n_acc = 0, n_att = 0;
     uint32_t lost = 0, exh = 0;
@@ -401,8 +400,9 @@ __global__ __launch_bounds__(256) void walk_kernel(const WalkArgs a) {
 
 template <class Rng>
 hipError_t launch_rng(const WalkArgs &a, int field, int prec, hipStream_t s) {
-    const dim3 block(256), grid((a.n_local + 255) / 256);
-    if (grid.x == 0) return hipSuccess;
+    const uint32_t nb = (a.n_local + 255) / 256;
+    if (nb == 0) return hipSuccess;
+    const dim3 block(256), grid((nb + 7) / 8 * 8);
 #define SW_LAUNCH(F, P) hipLaunchKernelGGL((walk_kernel<Rng, F, P>), grid, block, 0, s, a)
     if (prec == 2) {
         if (field == 0) SW_LAUNCH(FieldNone, PhaseF64);

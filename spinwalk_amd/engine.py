@@ -71,6 +71,17 @@ class SwWalkOpts(C.Structure):
     ]
 
 
+class SwShapeSpec(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("label", C.c_uint32), ("radius_um", C.c_float), ("volume_fraction", C.c_float),
+        ("random_orientation", C.c_int32), ("orientation_deg", C.c_float), ("dchi", C.c_float), ("Y", C.c_float),
+        ("seed", C.c_uint32), ("trunc_radii", C.c_float),
+    ]
+
+
+SW_SHAPE_CYLINDER, SW_SHAPE_SPHERE = 0, 1
+
+
 class SwWalkStats(C.Structure):
     _fields_ = [
         ("n_lost", C.c_uint64), ("n_replay_exhausted", C.c_uint64),
@@ -86,7 +97,7 @@ EXPORTS = (
     "sw_engine_create", "sw_engine_destroy", "sw_last_error", "sw_device_info", "sw_device_count",
     "sw_version", "sw_params_prepare", "sw_timing_scale", "sw_tesla2deg", "sw_init_positions",
     "sw_replay_fill", "sw_phantom_create", "sw_phantom_create_device", "sw_phantom_destroy",
-    "sw_phantom_info", "sw_sequence_create", "sw_sequence_destroy", "sw_walk_device", "sw_walk_host",
+    "sw_phantom_info", "sw_phantom_download", "sw_phantom_generate", "sw_sequence_create", "sw_sequence_destroy", "sw_walk_device", "sw_walk_host",
     "sw_echo_reduce_device", "sw_philox_draws_device", "sw_stream_sync",
 )
 
@@ -126,6 +137,8 @@ def load():
     L.sw_phantom_destroy.argtypes = [_P]
     L.sw_phantom_destroy.restype = None
     L.sw_phantom_info.argtypes = [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]
+    L.sw_phantom_download.argtypes = [_P, _P, _P]
+    L.sw_phantom_generate.argtypes = [_P, _P, _P, C.POINTER(SwShapeSpec), C.c_uint32, C.c_uint32, C.POINTER(_P), _P]
     L.sw_sequence_create.argtypes = [_P, C.POINTER(SwParams), C.POINTER(SwSeq), C.POINTER(_P)]
     L.sw_sequence_destroy.argtypes = [_P]
     L.sw_sequence_destroy.restype = None
@@ -364,7 +377,31 @@ class Engine:
         h = _P()
         self._check(self.L.sw_phantom_create(self.h, _np_ptr(mask), _np_ptr(ft), _np_ptr(dims), _np_ptr(fov),
                                              bits, C.byref(h)), "sw_phantom_create")
-        return Phantom(self, h)
+        ph = Phantom(self, h)
+        ph.dims = tuple(int(d) for d in dims)
+        ph.fov_m = fov
+        return ph
+
+    def generate_phantom(self, dims, fov_um, specs, formats=(SW_FIELD_F32,)):
+        """Device phantom generator.  specs: list of dicts with SwShapeSpec fields.
+        Returns (Phantom, analytic volume fraction placed per spec, percent)."""
+        arr = (SwShapeSpec * len(specs))()
+        for i, sp in enumerate(specs):
+            for k, v in sp.items():
+                setattr(arr[i], k, v)
+        dims = np.asarray(dims, np.uint64)
+        fov = np.ascontiguousarray(fov_um, np.float32)
+        vf = np.zeros(len(specs), np.float64)
+        bits = 0
+        for f in formats:
+            bits |= 1 << f
+        h = _P()
+        self._check(self.L.sw_phantom_generate(self.h, _np_ptr(dims), _np_ptr(fov), arr, len(specs), bits,
+                                               C.byref(h), _np_ptr(vf)), "sw_phantom_generate")
+        ph = Phantom(self, h)
+        ph.dims = tuple(int(d) for d in dims)
+        ph.fov_m = (fov * np.float32(1e-6)).astype(np.float32)
+        return ph, vf
 
     def phantom_device(self, d_mask, d_field, dims, fov, formats=(SW_FIELD_F32,)):
         dims = np.asarray(dims, np.uint64)
@@ -450,6 +487,13 @@ class Engine:
 class Phantom:
     def __init__(self, eng, h):
         self.eng, self.h = eng, h
+
+    def download(self, field=True):
+        dims = self.dims
+        mask = np.zeros(dims, np.uint8)
+        f = np.zeros(dims, np.float32) if field else None
+        self.eng._check(self.eng.L.sw_phantom_download(self.h, _np_ptr(mask), _np_ptr(f)), "sw_phantom_download")
+        return mask, f
 
     def info(self):
         n, s = C.c_uint32(), C.c_float()

@@ -1,0 +1,80 @@
+"""The benchmark workloads: synthetic phantoms and sequences of the five
+configurations the headline metric is quoted on, in the reference's INI key
+names and units (config/config_default.ini:1-127).
+
+Sequences follow the reference's config generator defaults
+(src/config/config_generator.cpp:92-174): GRE TR = TE + dt with a 90 degree
+excitation; SE 90/180 with phases 0/90 and the refocusing pulse at TE/2; bSSFP
+TR = 2 TE with FA 16.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import engine as E
+
+DCHI = 0.11e-6 * 0.4          # susceptibility difference x hematocrit
+Y_OXY = 0.7
+
+
+def cylinders(radius_um, vf, seed, label=1, random_orientation=1, orientation_deg=90.0):
+    return dict(kind=E.SW_SHAPE_CYLINDER, label=label, radius_um=radius_um, volume_fraction=vf,
+                random_orientation=random_orientation, orientation_deg=orientation_deg, dchi=DCHI, Y=Y_OXY,
+                seed=seed, trunc_radii=0.0)
+
+
+def spheres(radius_um, vf, seed, label=2):
+    return dict(kind=E.SW_SHAPE_SPHERE, label=label, radius_um=radius_um, volume_fraction=vf,
+                random_orientation=0, orientation_deg=0.0, dchi=DCHI, Y=Y_OXY, seed=seed, trunc_radii=10.0)
+
+
+def _base(n_sub, **kw):
+    cfg = dict(B0=9.4, SEED=10, CROSS_FOV=0, RECORD_TRAJECTORY=0, MAX_ITERATIONS=1e4, DUMMY_SCAN=0,
+               LINEAR_PHASE_CYCLING=0.0, QUADRATIC_PHASE_CYCLING=0.0, WHAT_TO_SCALE=0, SCALE=[1.0],
+               DIFFUSIVITY=[1e-9] * n_sub, P_XY=np.eye(n_sub).tolist())
+    cfg.update(kw)
+    return cfg
+
+
+def c1():
+    """100^3 over 100 um, random cylinders r 5 um at 4 %, GRE TE 20 ms dt 50 us, 1e5 spins."""
+    phantom = dict(dims=(100, 100, 100), fov_um=(100.0, 100.0, 100.0), specs=[cylinders(5.0, 4.0, seed=1)])
+    cfg = _base(2, TR=20050, TE=[20000], RF_FA=[90.0], RF_PH=[0.0], RF_T=[0], TIME_STEP=50,
+                NUMBER_OF_SPINS=100000, T1=[2200.0, 2200.0], T2=[40.0, 40.0])
+    return dict(name="c1_gre_100cube_1e5", phantom=phantom, cfg=cfg)
+
+
+def c2():
+    """500^3 over 500 um, random cylinders r 5 um at 4 % (label 1) + spheres r 3 um
+    at 2 % (label 2), P_XY 0.5 between substrates, T1/T2 2000/40 (tissue) and
+    1600/20 (inclusions), SE 90/180 TE 40 ms dt 20 us, 1e6 spins."""
+    phantom = dict(dims=(500, 500, 500), fov_um=(500.0, 500.0, 500.0),
+                   specs=[cylinders(5.0, 4.0, seed=1), spheres(3.0, 2.0, seed=2)])
+    pxy = [[1.0, 0.5, 0.5], [0.5, 1.0, 0.5], [0.5, 0.5, 1.0]]
+    cfg = _base(3, TR=40020, TE=[40000], RF_FA=[90.0, 180.0], RF_PH=[0.0, 90.0], RF_T=[0, 20000], TIME_STEP=20,
+                NUMBER_OF_SPINS=1000000, T1=[2000.0, 1600.0, 1600.0], T2=[40.0, 20.0, 20.0], P_XY=pxy)
+    return dict(name="c2_se_500cube_1e6", phantom=phantom, cfg=cfg)
+
+
+WORKLOADS = {"c1": c1, "c2": c2}
+
+
+def build(engine, wl, formats=(E.SW_FIELD_F32,), n_spins=None):
+    """Generate the phantom on the device and prepare the case (host transforms
+    through the engine).  Returns (phantom, case, vf)."""
+    ph, vf = engine.generate_phantom(wl["phantom"]["dims"], wl["phantom"]["fov_um"], wl["phantom"]["specs"],
+                                     formats=formats)
+    cfg = dict(wl["cfg"])
+    if n_spins is not None:
+        cfg["NUMBER_OF_SPINS"] = n_spins
+    n_labels, _ = ph.info()
+    dims = wl["phantom"]["dims"]
+    # the case needs the mask only for its shape on this path; the phantom stays on the device
+    case = E.prepare_case(cfg, np.zeros((1, 1, 1), np.uint8), None, ph.fov_m, XYZ0=None)
+    p = case.params
+    for i in range(3):
+        p.phantom_size[i] = dims[i]
+    p.matrix_length = int(np.prod(dims))
+    p.fieldmap_exist = 1
+    case.mask = None
+    return ph, case, vf

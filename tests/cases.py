@@ -159,60 +159,20 @@ def pgse(n_sub, b=1000.0, delta_ms=2.0, DELTA_ms=4.0, dt=50, axis=(1.0, 0.0, 0.0
 # engine Case -> oracle inputs
 # ---------------------------------------------------------------------------
 def to_oracle(case, scale_index=0, check_transforms=True):
-    """Oracle input dict for one scale of an engine Case (monte_carlo.cu:241-320)."""
-    p = case.params
-    raw = case.raw
-    dt = int(np.float32(raw["TIME_STEP"]))
-    TR = int(np.float32(raw["TR"]))
-    tabs = {}
-    for key, name in (("TE", "TE_t"), ("RF_T", "RF_t"), ("DEPHASING_T", "dephasing_t"), ("GRADIENT_T", "gradient_t")):
-        v = orc.timing_scale([int(np.float32(x)) for x in raw.get(key, [])], dt) if raw.get(key) else \
-            np.zeros(0, np.int32)
-        tabs[name] = v
-        if check_transforms:
-            assert np.array_equal(v, case.tables[name]), name
-    diff = np.asarray([float(np.float32(x)) for x in raw["DIFFUSIVITY"]], np.float64)
-    prep = orc.prepare(np.float32(raw["RF_FA"][0]), TR, dt, np.float32(raw["T1"][0]), diff, int(raw.get("DUMMY_SCAN", 0)))
+    """Oracle input dict for one scale of an engine Case, checking that the
+    engine's host transforms equal the oracle's restatement bit for bit."""
+    d = orc.from_engine_case(case, scale_index)
     if check_transforms:
+        p, prep = case.params, d["prep"]
+        for name in ("TE_t", "RF_t", "dephasing_t", "gradient_t"):
+            assert np.array_equal(d[name], case.tables[name]), name
         assert prep["c"] == p.c and prep["s"] == p.s
         assert prep["n_timepoints"] == p.n_timepoints
         assert prep["n_dummy_scan"] == p.n_dummy_scan
         assert np.array_equal(prep["diffusivity"], case.tables["diffusivity"])
-    scale = np.float32(case.scales[scale_index])
-    st = case.scale_type
-    fov = case.fov.astype(np.float32)
-    if st == 0:
-        fov_s = (scale * fov).astype(np.float32)                       # monte_carlo.cu:280
-        xyz0 = (case.XYZ0 * scale).astype(np.float32)                  # monte_carlo.cu:278
-    else:
-        fov_s, xyz0 = fov, case.XYZ0
-    d = dict(
-        fov=[float(x) for x in fov_s], phantom_size=list(case.mask.shape),
-        c=prep["c"], s=prep["s"],
-        linear_phase_cycling=float(np.float32(raw.get("LINEAR_PHASE_CYCLING", 0.0)) * (scale if st == 2 else np.float32(1))),
-        quadratic_phase_cycling=float(np.float32(raw.get("QUADRATIC_PHASE_CYCLING", 0.0))),
-        timestep_us=dt, TR_us=TR, n_dummy_scan=prep["n_dummy_scan"], n_spins=case.n_spins,
-        n_timepoints=prep["n_timepoints"], seed=int(p.seed), max_iterations=int(p.max_iterations),
-        cross_fov=int(p.enCrossFOV), record_trajectory=int(p.enRecordTrajectory),
-        diffusivity=prep["diffusivity"], T1_ms=np.asarray(raw["T1"], np.float32),
-        T2_ms=np.asarray(raw["T2"], np.float32), pXY=np.asarray(raw["P_XY"], np.float32).ravel(),
-        RF_FA_deg=np.asarray(raw["RF_FA"], np.float32), RF_PH_deg=np.asarray(raw["RF_PH"], np.float32),
-        RF_t=tabs["RF_t"], TE_t=tabs["TE_t"],
-        dephasing_deg=np.asarray(raw.get("DEPHASING", []), np.float32), dephasing_t=tabs["dephasing_t"],
-        gx=np.asarray(raw.get("GRADIENT_X", []), np.float32), gy=np.asarray(raw.get("GRADIENT_Y", []), np.float32),
-        gz=np.asarray(raw.get("GRADIENT_Z", []), np.float32), gradient_t=tabs["gradient_t"],
-        gradient_scale=float(scale) if st == 1 else 1.0,
-        mask=case.mask, M0=case.M0, XYZ0_scaled=xyz0, B0=float(np.float32(raw.get("B0", 9.4))),
-        field_T=case.field_T,
-    )
-    if case.field_T is not None:
-        k = orc.tesla2deg(d["B0"], dt)
-        if check_transforms:
+        if case.field_T is not None:
             from spinwalk_amd import engine as E
-            assert k == E.tesla2deg(d["B0"], dt)
-        d["field_deg"] = orc.scale_field(case.field_T, k)
-    else:
-        d["field_deg"] = None
+            assert orc.tesla2deg(d["B0"], d["timestep_us"]) == E.tesla2deg(d["B0"], d["timestep_us"])
     return d
 
 
