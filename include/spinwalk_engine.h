@@ -229,15 +229,6 @@ SW_API int  sw_philox_draws_device(sw_engine *eng, uint64_t seed, uint32_t spin0
 /* Waits for `stream` (NULL: the engine's stream). */
 SW_API int  sw_stream_sync(sw_engine *eng, void *stream);
 
-/* ---- driver: monte_carlo::run (src/sim/monte_carlo.cu:199-355) --------------
- * Implemented in libspinwalk_host.so (C++ host side: INI config, HDF5 I/O).
- * Declared here so that a caller binds one header. */
-SW_API int  sw_run_config(sw_engine *eng, const char *config_path);
-/* sim::handler::execute (src/sim/handler.cu:8-17).  use_cpu != 0 is rejected with
- * SW_E_INVALID: this engine has no CPU path (the reference's `-p` switch). */
-SW_API int  sw_sim_execute(int use_cpu, uint32_t device_id, const char *const *config_files,
-                           uint32_t n_files);
-
 #ifdef __cplusplus
 }
 #endif

@@ -678,6 +678,7 @@ SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence 
     a.h2d = ph->half_scale * a.t2d;
     a.t2d_f64 = (double)p->B0 * p->timestep_us * 1e-6 * kGamma * kRad2Deg;
     a.nx = (int64_t)ph->dims[0]; a.ny = (int64_t)ph->dims[1]; a.nz = (int64_t)ph->dims[2];
+    for (int i = 0; i < 3; i++) a.nd[i] = (double)ph->dims[i];
     a.matrix_length = p->matrix_length;
     for (int i = 0; i < 3; i++) {
         a.fov[i] = p->fov[i];

@@ -32,6 +32,8 @@ constexpr uint32_t kNever = 0xFFFFFFFFu;
 
 enum : int { ST_OK = 0, ST_LOST = 1, ST_EXHAUSTED = 2 };
 
+
+
 // ---------------------------------------------------------------------------
 // Rotations (src/sim/kernels.cuh:82-152), generic over float/double.
 // ---------------------------------------------------------------------------
@@ -122,25 +124,30 @@ struct PhaseF64 {
 // factor in float once on the host (monte_carlo.cu:243-244); multiplying the
 // gathered value by the same float factor gives the same float.
 // ---------------------------------------------------------------------------
-struct FieldNone { template <class R> __device__ static R get(const WalkArgs &, int64_t) { return (R)0; } };
-struct FieldF32 {
-    template <class R> __device__ static R get(const WalkArgs &a, int64_t i);
+template <class T, bool IDX32>
+__device__ __forceinline__ T load_at(const T *base, int64_t i) {
+    if (IDX32) return *(const T *)((const char *)base + (uint32_t)((uint32_t)i * (uint32_t)sizeof(T)));
+    return base[i];
+}
+
+struct FieldNone {
+    template <class R, bool IDX32> __device__ static R get(const WalkArgs &, int64_t) { return (R)0; }
 };
-template <> __device__ float FieldF32::get<float>(const WalkArgs &a, int64_t i) {
-    return a.field_f32[i] * a.t2d;
-}
-template <> __device__ double FieldF32::get<double>(const WalkArgs &a, int64_t i) {
-    return (double)a.field_f32[i] * a.t2d_f64;
-}
+struct FieldF32 {
+    template <class R, bool IDX32> __device__ static R get(const WalkArgs &a, int64_t i) {
+        if (sizeof(R) == 8) return (R)((double)load_at<float, IDX32>(a.field_f32, i) * a.t2d_f64);
+        return (R)(load_at<float, IDX32>(a.field_f32, i) * a.t2d);
+    }
+};
 struct FieldF16 {
-    template <class R> __device__ static R get(const WalkArgs &a, int64_t i) {
-        const __half h = __ushort_as_half(a.field_h[i]);
+    template <class R, bool IDX32> __device__ static R get(const WalkArgs &a, int64_t i) {
+        const __half h = __ushort_as_half(load_at<uint16_t, IDX32>(a.field_h, i));
         return (R)(__half2float(h) * a.h2d);
     }
 };
 struct FieldBF16 {
-    template <class R> __device__ static R get(const WalkArgs &a, int64_t i) {
-        const float f = __uint_as_float((uint32_t)a.field_h[i] << 16);
+    template <class R, bool IDX32> __device__ static R get(const WalkArgs &a, int64_t i) {
+        const float f = __uint_as_float((uint32_t)load_at<uint16_t, IDX32>(a.field_h, i) << 16);
         return (R)(f * a.h2d);
     }
 };
@@ -148,20 +155,25 @@ struct FieldBF16 {
 // ---------------------------------------------------------------------------
 // Draw sources.
 // ---------------------------------------------------------------------------
+// Both sources are used as peek/commit: peek() makes the normals of the next
+// attempt available ahead of time (so they can be computed while the current
+// attempt's gathers are in flight); commit() consumes them.  An attempt that does
+// not move (diffusivity 0) consumes nothing, as in the reference (kernels.cu:130).
 struct ReplayRng {          // the reference CPU path's recorded streams
     const float *n, *u;
     uint64_t nn, nu, in = 0, iu = 0;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    bool avail = false;
     __device__ ReplayRng(const WalkArgs &a, uint32_t li, uint32_t) {
         n = a.rn + (uint64_t)li * a.rn_stride; nn = a.rn_stride;
         u = a.ru + (uint64_t)li * a.ru_stride; nu = a.ru_stride;
     }
-    __device__ bool normals(float &x, float &y, float &z) {
-        if (in + 3 > nn) return false;
-        x = n[in]; y = n[in + 1]; z = n[in + 2];
-        in += 3;
-        return true;
+    __device__ void peek() {
+        avail = in + 3 <= nn;
+        if (avail) { px = n[in]; py = n[in + 1]; pz = n[in + 2]; }
     }
-    __device__ bool uniform(float &v) {
+    __device__ void commit() { in += 3; }
+    __device__ bool uniform(float &v) {       // gen_u stream, one draw per substrate change
         if (iu >= nu) return false;
         v = u[iu++];
         return true;
@@ -171,57 +183,74 @@ struct ReplayRng {          // the reference CPU path's recorded streams
 struct PhiloxRng {          // production: Philox4x32-10, one block per attempted step
     uint32_t k0, k1, spin;
     uint64_t attempt = 0;
-    float upend = 0.f;
+    float px = 0.f, py = 0.f, pz = 0.f, pu = 0.f, ucur = 0.f;
+    static constexpr bool avail = true;
     __device__ PhiloxRng(const WalkArgs &a, uint32_t, uint32_t spin_global)
         : k0((uint32_t)a.seed), k1((uint32_t)(a.seed >> 32)), spin(spin_global) {}
-    __device__ bool normals(float &x, float &y, float &z) {
+    __device__ void peek() {
         const U32x4 r = philox4x32_10((uint32_t)attempt, (uint32_t)(attempt >> 32), spin, 0u, k0, k1);
-        ++attempt;
         // Box-Muller on (r0,r1) and (r2,r3); v_log_f32 is log2, v_sin/v_cos take revolutions
         const float u1 = u24_open(r.v[0]), v1 = u24_closed0(r.v[1]);
         const float u2 = u24_open(r.v[2]), v2 = u24_closed0(r.v[3]);
         const float rho1 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
         const float rho2 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u2));
-        x = rho1 * __builtin_amdgcn_cosf(v1);
-        y = rho1 * __builtin_amdgcn_sinf(v1);
-        z = rho2 * __builtin_amdgcn_cosf(v2);
-        upend = u24_lowbytes(r.v[0], r.v[1], r.v[2]);
-        return true;
+        px = rho1 * __builtin_amdgcn_cosf(v1);
+        py = rho1 * __builtin_amdgcn_sinf(v1);
+        pz = rho2 * __builtin_amdgcn_cosf(v2);
+        pu = u24_lowbytes(r.v[0], r.v[1], r.v[2]);
     }
-    __device__ bool uniform(float &v) { v = upend; return true; }
+    __device__ void commit() { ucur = pu; ++attempt; }
+    // the permeability uniform of the attempt last committed
+    __device__ bool uniform(float &v) { v = ucur; return true; }
 };
 
 // ---------------------------------------------------------------------------
 // sub2ind on truncated scaled positions (kernels.cu:140, kernels.cuh:53-60).
+// When every scaled coordinate lies in [0, n) the truncations are the
+// subscripts and the row-major index is computed in 32 bits (IDX32: nx*ny < 2^24
+// and 4*matrix_length < 2^32) with 24-bit multiplies.  Anything else takes the
+// reference's int64 arithmetic, including its behaviour for a subscript outside
+// its axis (the index may still land inside the volume, in a neighbouring row).
 // ---------------------------------------------------------------------------
+template <bool IDX32>
 __device__ __forceinline__ int64_t voxel_index(const WalkArgs &a, const double x[3]) {
     const double vx = x[0] * a.s2g[0], vy = x[1] * a.s2g[1], vz = x[2] * a.s2g[2];
-    constexpr double kLim = 2147483648.0;
-    if (fabs(vx) < kLim && fabs(vy) < kLim && fabs(vz) < kLim) {
-        // v_cvt_i32_f64 truncates toward zero like the double->int64 conversion
-        const int64_t ix = (int32_t)vx, iy = (int32_t)vy, iz = (int32_t)vz;
-        return ix * a.nz * a.ny + iy * a.nz + iz;
+    if (vx >= 0.0 && vx < a.nd[0] && vy >= 0.0 && vy < a.nd[1] && vz >= 0.0 && vz < a.nd[2]) {
+        const uint32_t ix = (uint32_t)vx, iy = (uint32_t)vy, iz = (uint32_t)vz;
+        if (IDX32)
+            return (int64_t)(__umul24(__umul24(ix, (uint32_t)a.ny) + iy, (uint32_t)a.nz) + iz);
+        return ((int64_t)ix * a.ny + iy) * a.nz + iz;
     }
+    constexpr double kLim = 9.2e18;   // beyond int64: the conversion is undefined; treat as outside
+    if (!(fabs(vx) < kLim && fabs(vy) < kLim && fabs(vz) < kLim)) return -1;
     const int64_t ix = (int64_t)vx, iy = (int64_t)vy, iz = (int64_t)vz;
     return ix * a.nz * a.ny + iy * a.nz + iz;
 }
 
-__device__ __forceinline__ void move_axis(const WalkArgs &a, int i, float r, double diff,
-                                          const double xo[3], double xn[3]) {
+template <bool CROSS>
+__device__ __forceinline__ double move_axis(double fov, float r, double diff, double xo) {
     const double rw = (double)r * diff;                          // kernels.cu:131
-    double v = xo[i] + rw;
-    if (v < 0)
-        v += (a.cross_fov ? a.fov[i] : 2 * fabs(rw));            // :133-134
-    else if (v >= a.fov[i])
-        v -= (a.cross_fov ? a.fov[i] : 2 * fabs(rw));            // :135-136
-    xn[i] = v;
+    const double v = xo + rw;
+    const double adj = CROSS ? fov : 2 * fabs(rw);               // :133-136
+    return v < 0 ? v + adj : (v >= fov ? v - adj : v);
+}
+
+// Field gather at a voxel: 32-bit byte offsets from the uniform base when IDX32.
+template <class Field, class R, bool IDX32>
+__device__ __forceinline__ R gather_field(const WalkArgs &a, int64_t ind) {
+    return Field::template get<R, IDX32>(a, ind);
 }
 
 // ---------------------------------------------------------------------------
-// One spin.  Returns ST_OK / ST_LOST / ST_EXHAUSTED and counts steps.
+// One spin.  Returns ST_OK / ST_LOST / ST_EXHAUSTED; counts accepted steps and
+// rejected attempts.
 // ---------------------------------------------------------------------------
-template <class Rng, class Field, class Phase>
-__device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint64_t &n_att) {
+__device__ __forceinline__ uint32_t next_event(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    return min(min(a, b), min(c, d));
+}
+
+template <class Rng, class Field, class Phase, bool CROSS, bool IDX32>
+__device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32_t &n_rej) {
     using R = typename Phase::R;
     const uint32_t spin = a.spin_offset + li;
     float *xyz1 = a.XYZ1 + 3 * (uint64_t)li * a.traj_len;
@@ -233,20 +262,27 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint64
         m[i] = (R)a.M0[3 * (uint64_t)li + i];
         if (a.record_traj) xyz1[i] = x0;
     }
-    int64_t ind = voxel_index(a, xo);                              // kernels.cu:102
+    int64_t ind = voxel_index<IDX32>(a, xo);                       // kernels.cu:102
     if (ind < 0 || ind >= a.matrix_length) {                       // the reference reads out of bounds here
         if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
         return ST_LOST;
     }
-    uint8_t ts_old = a.mask[ind];                                  // :103
+    uint8_t ts_old = load_at<uint8_t, IDX32>(a.mask, ind);         // :103
     double diff = a.diffusivity[ts_old];                           // :104
+    // The reference re-reads T1/T2/diffusivity of ts_old at every voxel change
+    // (:167-169); they only change with ts_old, so they are read when it changes.
+    // Before the first accepted step T1 = T2 = 0 (:91); no event can use them
+    // before that step unless the TR has no time points at all.
     R field = 0, T1 = 0, T2 = 0;
+    if (a.n_timepoints > 0) {
+        if (sizeof(R) == 8) { T1 = (R)a.T1d[ts_old]; T2 = (R)a.T2d[ts_old]; }
+        else                { T1 = (R)a.T1s[ts_old]; T2 = (R)a.T2s[ts_old]; }
+    }
     uint32_t itr = 0;
     Rng rng(a, li, spin);
+    rng.peek();
     Phase ph;
     uint64_t slot = 0;
-    const int64_t nzny = a.nz * a.ny;
-    (void)nzny;
 
     for (uint32_t d = 0; d <= a.n_dummy; ++d) {                    // :107
         const bool last = (d == a.n_dummy);
@@ -264,84 +300,100 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint64
         uint32_t next_te = (last && a.n_te > 0) ? (uint32_t)a.TE_t[0] : kNever;
         uint32_t next_dp = (a.n_deph > 0) ? (uint32_t)a.deph_t[0] : kNever;
         uint32_t next_gr = (a.n_grad > 0) ? (uint32_t)a.grad_t[0] : kNever;
+        uint32_t next_ev = next_event(next_rf, next_te, next_dp, next_gr);
         ph.reset();
 
         while (tp < a.n_timepoints) {                              // :128
-            if (diff != 0.) {                                      // :130-137
-                float r0, r1, r2;
-                if (!rng.normals(r0, r1, r2)) return ST_EXHAUSTED;
-                move_axis(a, 0, r0, diff, xo, xn);
-                move_axis(a, 1, r1, diff, xo, xn);
-                move_axis(a, 2, r2, diff, xo, xn);
+            const bool moved = diff != 0.;
+            if (moved) {                                           // :130-137
+                if (!rng.avail) { n_acc += (uint64_t)d * a.n_timepoints + tp; return ST_EXHAUSTED; }
+                xn[0] = move_axis<CROSS>(a.fov[0], rng.px, diff, xo[0]);
+                xn[1] = move_axis<CROSS>(a.fov[1], rng.py, diff, xo[1]);
+                xn[2] = move_axis<CROSS>(a.fov[2], rng.pz, diff, xo[2]);
             }
-            ++n_att;
-            ind = voxel_index(a, xn);                              // :140
+            ind = voxel_index<IDX32>(a, xn);                       // :140
             if (ind >= a.matrix_length || ind < 0) {               // :141-147
                 if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
+                n_acc += (uint64_t)d * a.n_timepoints + tp;
                 return ST_LOST;
             }
-            if (ind != ind_old) {                                  // :150
-                const uint8_t ts = a.mask[ind];                    // :152
+            const bool newvox = ind != ind_old;                    // :150
+            uint8_t ts = ts_old;
+            R fnew = field;
+            if (newvox) {                                          // issue the gathers first
+                ts = load_at<uint8_t, IDX32>(a.mask, ind);         // :152
+                fnew = gather_field<Field, R, IDX32>(a, ind);      // :166, read speculatively
+            }
+            if (moved) {                                           // next attempt's draws while loads fly
+                rng.commit();
+                rng.peek();
+            }
+            if (newvox) {
                 if (ts != ts_old) {                                // :153-163
                     float u;
-                    if (!rng.uniform(u)) return ST_EXHAUSTED;
+                    if (!rng.uniform(u)) { n_acc += (uint64_t)d * a.n_timepoints + tp; return ST_EXHAUSTED; }
                     if (u >= a.pXY[ts_old * a.n_sub + ts]) {
+                        ++n_rej;
                         if (itr++ > a.max_iterations) {
                             if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
+                            n_acc += (uint64_t)d * a.n_timepoints + tp;
                             return ST_LOST;
                         }
                         continue;
                     }
                     ts_old = ts;
+                    if (sizeof(R) == 8) { T1 = (R)a.T1d[ts]; T2 = (R)a.T2d[ts]; }
+                    else                { T1 = (R)a.T1s[ts]; T2 = (R)a.T2s[ts]; }
+                    diff = a.diffusivity[ts];
                 }
-                ind_old = ind;                                     // :165-169
-                field = Field::template get<R>(a, ind);
-                if (sizeof(R) == 8) { T1 = (R)a.T1d[ts_old]; T2 = (R)a.T2d[ts_old]; }
-                else                { T1 = (R)a.T1s[ts_old]; T2 = (R)a.T2s[ts_old]; }
-                diff = a.diffusivity[ts_old];
+                ind_old = ind;                                     // :165
+                field = fnew;
             }
             ph.add(field);                                         // :171
             itr = 0;
 
-            if (tp == next_dp) {                                   // :175-178
-                if (sizeof(R) == 8)
-                    ph.add((R)((double)spin * a.deph_deg[dp_i] / (double)a.n_spins_total));
-                else
-                    ph.add((R)((float)spin * a.deph_deg[dp_i] / (float)a.n_spins_total));
-                ++dp_i;
-                next_dp = (dp_i < a.n_deph) ? (uint32_t)a.deph_t[dp_i] : kNever;
-            }
-            if (tp == next_gr) {                                   // :181-187
-                const float Gx = a.gx[gr_i] * a.gradient_scale;
-                const float Gy = a.gy[gr_i] * a.gradient_scale;
-                const float Gz = a.gz[gr_i] * a.gradient_scale;
-                ph.add_grad((Gx * xn[0] + Gy * xn[1] + Gz * xn[2]) * 1e-3 * a.timestep_us * 1e-6 *
-                            kGamma * kRad2Deg);
-                ++gr_i;
-                next_gr = (gr_i < a.n_grad) ? (uint32_t)a.grad_t[gr_i] : kNever;
-            }
-            if (tp == next_rf) {                                   // :190-199
-                const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
-                dephase_relax(m, ph.value(), T1, T2, te);
-                const RfPulse p = a.rf[rf_i];
-                if (sizeof(R) == 8) rot_apply<R>(p.kind_d, (R)p.sd, (R)p.cd, (R)p.spd, (R)p.cpd, m);
-                else                rot_apply<R>(p.kind, (R)p.s, (R)p.c, (R)p.sp, (R)p.cp, m);
-                ph.reset();
-                old_tp = tp;
-                ++rf_i;
-                next_rf = (rf_i < a.n_rf) ? (uint32_t)a.rf[rf_i].t : kNever;
-            }
-            if (tp == next_te) {                                   // :202-215
-                const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
-                dephase_relax(m, ph.value(), T1, T2, te);
-                const uint64_t o = (uint64_t)li * a.n_te + te_i;
-                R *M1 = (R *)a.M1;
-                M1[3 * o + 0] = m[0]; M1[3 * o + 1] = m[1]; M1[3 * o + 2] = m[2];
-                a.T[o] = ts_old;
-                ph.reset();
-                old_tp = tp;
-                ++te_i;
-                next_te = (te_i < a.n_te) ? (uint32_t)a.TE_t[te_i] : kNever;
+            if (tp == next_ev) {
+                if (tp == next_dp) {                               // :175-178
+                    if (sizeof(R) == 8)
+                        ph.add((R)((double)spin * a.deph_deg[dp_i] / (double)a.n_spins_total));
+                    else
+                        ph.add((R)((float)spin * a.deph_deg[dp_i] / (float)a.n_spins_total));
+                    ++dp_i;
+                    next_dp = (dp_i < a.n_deph) ? (uint32_t)a.deph_t[dp_i] : kNever;
+                }
+                if (tp == next_gr) {                               // :181-187
+                    const float Gx = a.gx[gr_i] * a.gradient_scale;
+                    const float Gy = a.gy[gr_i] * a.gradient_scale;
+                    const float Gz = a.gz[gr_i] * a.gradient_scale;
+                    ph.add_grad((Gx * xn[0] + Gy * xn[1] + Gz * xn[2]) * 1e-3 * a.timestep_us * 1e-6 *
+                                kGamma * kRad2Deg);
+                    ++gr_i;
+                    next_gr = (gr_i < a.n_grad) ? (uint32_t)a.grad_t[gr_i] : kNever;
+                }
+                if (tp == next_rf) {                               // :190-199
+                    const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
+                    dephase_relax(m, ph.value(), T1, T2, te);
+                    const RfPulse p = a.rf[rf_i];
+                    if (sizeof(R) == 8) rot_apply<R>(p.kind_d, (R)p.sd, (R)p.cd, (R)p.spd, (R)p.cpd, m);
+                    else                rot_apply<R>(p.kind, (R)p.s, (R)p.c, (R)p.sp, (R)p.cp, m);
+                    ph.reset();
+                    old_tp = tp;
+                    ++rf_i;
+                    next_rf = (rf_i < a.n_rf) ? (uint32_t)a.rf[rf_i].t : kNever;
+                }
+                if (tp == next_te) {                               // :202-215
+                    const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
+                    dephase_relax(m, ph.value(), T1, T2, te);
+                    const uint64_t o = (uint64_t)li * a.n_te + te_i;
+                    R *M1 = (R *)a.M1;
+                    M1[3 * o + 0] = m[0]; M1[3 * o + 1] = m[1]; M1[3 * o + 2] = m[2];
+                    a.T[o] = ts_old;
+                    ph.reset();
+                    old_tp = tp;
+                    ++te_i;
+                    next_te = (te_i < a.n_te) ? (uint32_t)a.TE_t[te_i] : kNever;
+                }
+                next_ev = next_event(next_rf, next_te, next_dp, next_gr);
             }
             if (a.record_traj) {                                   // :218-221
                 float *dst = xyz1 + 3 * slot;
@@ -350,12 +402,12 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint64
             }
             xo[0] = xn[0]; xo[1] = xn[1]; xo[2] = xn[2];
             ++tp;
-            ++n_acc;
         }
         const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);   // :226-231
         dephase_relax(m, ph.value(), T1, T2, te);
     }
     if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
+    n_acc += (uint64_t)(a.n_dummy + 1) * a.n_timepoints;
     return ST_OK;
 }
 
@@ -372,7 +424,7 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-template <class Rng, class Field, class Phase>
+template <class Rng, class Field, class Phase, bool CROSS, bool IDX32>
 __global__ __launch_bounds__(256) void walk_kernel(const WalkArgs a) {
     // Blocks are dealt round-robin over the 8 XCDs; remap so that each XCD gets a
     // contiguous range of the visiting order (spatially sorted spins share L2).
@@ -380,22 +432,35 @@ __global__ __launch_bounds__(256) void walk_kernel(const WalkArgs a) {
     const uint32_t per = gridDim.x / 8, b = blockIdx.x;
     const uint32_t lb = (b % 8) * per + b / 8;
     const uint32_t t = lb * blockDim.x + threadIdx.x;
-    uint64_t n_acc = 0, n_att = 0;
-    uint32_t lost = 0, exh = 0;
+    uint64_t n_acc = 0;
+    uint32_t n_rej = 0, lost = 0, exh = 0;
     if (t < a.n_local) {
         const uint32_t li = a.order ? a.order[t] : t;
-        const int st = walk_spin<Rng, Field, Phase>(a, li, n_acc, n_att);
+        const int st = walk_spin<Rng, Field, Phase, CROSS, IDX32>(a, li, n_acc, n_rej);
         lost = (st == ST_LOST);
         exh = (st == ST_EXHAUSTED);
     }
-    const uint64_t s_acc = wave_sum_u64(n_acc), s_att = wave_sum_u64(n_att);
+    const uint64_t s_acc = wave_sum_u64(n_acc), s_rej = wave_sum_u64(n_rej);
     const uint64_t s_lost = wave_sum_u64(lost), s_exh = wave_sum_u64(exh);
     if ((threadIdx.x & 63) == 0 && a.counters) {
         if (s_lost) atomicAdd(&a.counters[0], (unsigned long long)s_lost);
         if (s_exh)  atomicAdd(&a.counters[1], (unsigned long long)s_exh);
         atomicAdd(&a.counters[2], (unsigned long long)s_acc);
-        atomicAdd(&a.counters[3], (unsigned long long)s_att);
+        atomicAdd(&a.counters[3], (unsigned long long)(s_acc + s_rej));
     }
+}
+
+template <class Rng, class F, class P>
+hipError_t launch_fp(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
+    const bool idx32 = (uint64_t)a.nx * (uint64_t)a.ny < (1ull << 24) && (uint64_t)a.matrix_length * 4 < (1ull << 32);
+    if (a.cross_fov) {
+        if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, true>), grid, block, 0, s, a);
+        else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, false>), grid, block, 0, s, a);
+    } else {
+        if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, false, true>), grid, block, 0, s, a);
+        else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, false, false>), grid, block, 0, s, a);
+    }
+    return hipGetLastError();
 }
 
 template <class Rng>
@@ -403,28 +468,21 @@ hipError_t launch_rng(const WalkArgs &a, int field, int prec, hipStream_t s) {
     const uint32_t nb = (a.n_local + 255) / 256;
     if (nb == 0) return hipSuccess;
     const dim3 block(256), grid((nb + 7) / 8 * 8);
-#define SW_LAUNCH(F, P) hipLaunchKernelGGL((walk_kernel<Rng, F, P>), grid, block, 0, s, a)
     if (prec == 2) {
-        if (field == 0) SW_LAUNCH(FieldNone, PhaseF64);
-        else if (field == 1) SW_LAUNCH(FieldF32, PhaseF64);
-        else return hipErrorInvalidValue;
+        if (field == 0) return launch_fp<Rng, FieldNone, PhaseF64>(a, s, grid, block);
+        if (field == 1) return launch_fp<Rng, FieldF32, PhaseF64>(a, s, grid, block);
     } else if (prec == 1) {
-        if (field == 0) SW_LAUNCH(FieldNone, PhaseKahan);
-        else if (field == 1) SW_LAUNCH(FieldF32, PhaseKahan);
-        else if (field == 2) SW_LAUNCH(FieldF16, PhaseKahan);
-        else if (field == 3) SW_LAUNCH(FieldBF16, PhaseKahan);
-        else return hipErrorInvalidValue;
+        if (field == 0) return launch_fp<Rng, FieldNone, PhaseKahan>(a, s, grid, block);
+        if (field == 1) return launch_fp<Rng, FieldF32, PhaseKahan>(a, s, grid, block);
+        if (field == 2) return launch_fp<Rng, FieldF16, PhaseKahan>(a, s, grid, block);
+        if (field == 3) return launch_fp<Rng, FieldBF16, PhaseKahan>(a, s, grid, block);
     } else if (prec == 0) {
-        if (field == 0) SW_LAUNCH(FieldNone, PhaseF32);
-        else if (field == 1) SW_LAUNCH(FieldF32, PhaseF32);
-        else if (field == 2) SW_LAUNCH(FieldF16, PhaseF32);
-        else if (field == 3) SW_LAUNCH(FieldBF16, PhaseF32);
-        else return hipErrorInvalidValue;
-    } else {
-        return hipErrorInvalidValue;
+        if (field == 0) return launch_fp<Rng, FieldNone, PhaseF32>(a, s, grid, block);
+        if (field == 1) return launch_fp<Rng, FieldF32, PhaseF32>(a, s, grid, block);
+        if (field == 2) return launch_fp<Rng, FieldF16, PhaseF32>(a, s, grid, block);
+        if (field == 3) return launch_fp<Rng, FieldBF16, PhaseF32>(a, s, grid, block);
     }
-#undef SW_LAUNCH
-    return hipGetLastError();
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
@@ -459,10 +517,8 @@ __global__ void philox_draws_kernel(uint64_t seed, uint32_t spin0, uint32_t n_sp
     a.seed = seed;
     PhiloxRng r(a, 0, spin0 + i);
     r.attempt = k;
-    float x, y, z, u;
-    r.normals(x, y, z);
-    r.uniform(u);
-    out[4 * g + 0] = x; out[4 * g + 1] = y; out[4 * g + 2] = z; out[4 * g + 3] = u;
+    r.peek();
+    out[4 * g + 0] = r.px; out[4 * g + 1] = r.py; out[4 * g + 2] = r.pz; out[4 * g + 3] = r.pu;
 }
 
 hipError_t launch_philox_draws(uint64_t seed, uint32_t spin0, uint32_t n_spins, uint32_t n_att,

@@ -40,6 +40,7 @@ struct WalkArgs {
     float           h2d;          // half -> deg/step factor = half_scale * t2d
     double          t2d_f64;      // double T -> deg/step factor for the FP64 path
     int64_t         nx, ny, nz, matrix_length;
+    double          nd[3];        // nx, ny, nz as doubles
     double          fov[3];
     double          s2g[3];       // phantom_size / fov (kernels.cu:98)
 

@@ -182,3 +182,33 @@ def make_case(cfg, mask, field_T, fov_um, n_spins=None, xyz0=None):
         cfg = dict(cfg, NUMBER_OF_SPINS=n_spins)
     fov = np.full(3, np.float32(fov_um * 1e-6), np.float32)
     return E.prepare_case(cfg, mask, field_T, fov, XYZ0=xyz0)
+
+
+def write_config(path, cfg, phantoms, output_dir="out", seq_name="test", xyz0=None):
+    """Write `cfg` (reference key names) as an INI file the host reader accepts."""
+    from spinwalk_amd import host as H
+    files = {"OUTPUT_DIR": output_dir}
+    for i, p in enumerate(phantoms):
+        files[f"PHANTOM[{i}]"] = p
+    for i, p in enumerate(xyz0 or []):
+        files[f"XYZ0[{i}]"] = p
+    tissue = {}
+    for i, v in enumerate(cfg["DIFFUSIVITY"]):
+        tissue[f"DIFFUSIVITY[{i}]"] = repr(float(v))
+    for i, row in enumerate(np.asarray(cfg["P_XY"], float)):
+        tissue[f"P_XY[{i}]"] = " ".join(repr(float(x)) for x in row)
+    for i, v in enumerate(cfg["T1"]):
+        tissue[f"T1[{i}]"] = repr(float(v))
+    for i, v in enumerate(cfg["T2"]):
+        tissue[f"T2[{i}]"] = repr(float(v))
+    scan = {k: cfg[k] for k in ("TR", "TE", "RF_FA", "RF_PH", "RF_T", "TIME_STEP", "DUMMY_SCAN",
+                                "LINEAR_PHASE_CYCLING", "QUADRATIC_PHASE_CYCLING") if k in cfg}
+    for k in ("DEPHASING", "DEPHASING_T", "GRADIENT_X", "GRADIENT_Y", "GRADIENT_Z", "GRADIENT_T"):
+        if cfg.get(k) is not None and len(cfg.get(k)):
+            scan[k] = cfg[k]
+    sim = {k: cfg[k] for k in ("B0", "SEED", "NUMBER_OF_SPINS", "CROSS_FOV", "RECORD_TRAJECTORY", "MAX_ITERATIONS",
+                               "WHAT_TO_SCALE") if k in cfg}
+    for i, s in enumerate(cfg.get("SCALE", [1.0])):
+        sim[f"SCALE[{i}]"] = repr(float(s))
+    H.write_ini(path, {"GENERAL": {"SEQ_NAME": seq_name}, "FILES": files, "TISSUE_PARAMETERS": tissue,
+                       "SCAN_PARAMETERS": scan, "SIMULATION_PARAMETERS": sim})
