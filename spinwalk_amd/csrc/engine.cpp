@@ -55,11 +55,11 @@ struct sw_phantom {
     uint64_t n = 0;
     uint8_t *mask = nullptr;
     float *field = nullptr;          // tesla per 1 T
-    uint16_t *field_f16 = nullptr;
-    uint16_t *field_bf16 = nullptr;
     float half_scale = 1.f;
     uint32_t n_labels = 0;
     bool has_field = false;
+    // bricked voxel records gathered by the walk (walk_args.hpp)
+    void *rec[4] = {nullptr, nullptr, nullptr, nullptr};   // by SW_FIELD_*: none(label only), f32, f16, bf16
 };
 
 struct sw_sequence {
@@ -341,21 +341,21 @@ static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
     int ex = 0;
     if (fmax > 0 && std::isfinite(fmax)) { std::frexp(fmax, &ex); ph->half_scale = std::ldexp(1.0f, ex - 15); }
     else ph->half_scale = 1.f;
-    if (ph->has_field && (formats & ((1u << SW_FIELD_F16) | (1u << SW_FIELD_BF16)))) {
-        if (formats & (1u << SW_FIELD_F16)) {
-            SW_HIP(e, hipMalloc((void **)&ph->field_f16, ph->n * sizeof(uint16_t)));
-            SW_HIP(e, sw::launch_field_to_half(ph->field, ph->n, 1.0f / ph->half_scale, 0, ph->field_f16, s));
-        }
-        if (formats & (1u << SW_FIELD_BF16)) {
-            SW_HIP(e, hipMalloc((void **)&ph->field_bf16, ph->n * sizeof(uint16_t)));
-            SW_HIP(e, sw::launch_field_to_half(ph->field, ph->n, 1.0f / ph->half_scale, 1, ph->field_bf16, s));
-        }
-        SW_HIP(e, hipStreamSynchronize(s));
+    // bricked records for the walk
+    const uint64_t nx = ph->dims[0], ny = ph->dims[1], nz = ph->dims[2];
+    for (int f = 0; f < 4; f++) {
+        const bool want = (f == SW_FIELD_NONE) ? !ph->has_field : (ph->has_field && (formats & (1u << f)));
+        if (!want) continue;
+        const sw::BrickShape sh = f == SW_FIELD_F32 ? sw::kBrick8 : sw::kBrick4;
+        const size_t recsz = f == SW_FIELD_F32 ? 8 : 4;
+        const size_t nrec = (size_t)(sw::brick_count(nx, sh.lbx) * sw::brick_count(ny, sh.lby) *
+                                     sw::brick_count(nz, sh.lbz)) << (sh.lbx + sh.lby + sh.lbz);
+        SW_HIP(e, hipMalloc(&ph->rec[f], nrec * recsz));
+        SW_HIP(e, hipMemsetAsync(ph->rec[f], 0, nrec * recsz, s));
+        const int kind = f == SW_FIELD_F32 ? 0 : (f == SW_FIELD_F16 ? 1 : (f == SW_FIELD_BF16 ? 2 : 3));
+        SW_HIP(e, sw::launch_records(ph->mask, ph->field, nx, ny, nz, kind, 1.0f / ph->half_scale, ph->rec[f], s));
     }
-    if (ph->has_field && !(formats & (1u << SW_FIELD_F32))) {
-        dev_free(ph->field);
-        ph->field = nullptr;
-    }
+    SW_HIP(e, hipStreamSynchronize(s));
     return SW_OK;
 }
 
@@ -409,8 +409,7 @@ SW_API void sw_phantom_destroy(sw_phantom *ph) {
     (void)hipSetDevice(ph->device);
     dev_free(ph->mask);
     dev_free(ph->field);
-    dev_free(ph->field_f16);
-    dev_free(ph->field_bf16);
+    for (void *r : ph->rec) dev_free(r);
     delete ph;
 }
 
@@ -660,10 +659,8 @@ SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence 
         return fail(e, SW_E_INVALID, "replay mode needs normal and uniform streams");
     if (o->rng != SW_RNG_REPLAY && o->rng != SW_RNG_PHILOX) return fail(e, SW_E_INVALID, "unknown rng");
     int field = ph->has_field ? o->field_format : SW_FIELD_NONE;
-    if (field == SW_FIELD_F32 && !ph->field) return fail(e, SW_E_INVALID, "phantom has no float32 field copy");
-    if (field == SW_FIELD_F16 && !ph->field_f16) return fail(e, SW_E_INVALID, "phantom has no float16 field copy");
-    if (field == SW_FIELD_BF16 && !ph->field_bf16) return fail(e, SW_E_INVALID, "phantom has no bfloat16 field copy");
     if (field < 0 || field > 3) return fail(e, SW_E_INVALID, "unknown field format");
+    if (!ph->rec[field]) return fail(e, SW_E_INVALID, "the phantom was not created with this field format");
     if (o->precision == SW_PREC_F64 && field > SW_FIELD_F32)
         return fail(e, SW_E_INVALID, "the FP64 path gathers the float32 field");
     int rc = ensure_exc(e, q, p, s);
@@ -671,9 +668,12 @@ SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence 
 
     sw::WalkArgs a;
     std::memset(&a, 0, sizeof a);
-    a.mask = ph->mask;
-    a.field_f32 = ph->field;
-    a.field_h = field == SW_FIELD_F16 ? ph->field_f16 : ph->field_bf16;
+    a.rec = ph->rec[field];
+    {
+        const sw::BrickShape sh = field == SW_FIELD_F32 ? sw::kBrick8 : sw::kBrick4;
+        a.nb_y = (uint32_t)sw::brick_count(ph->dims[1], sh.lby);
+        a.nb_z = (uint32_t)sw::brick_count(ph->dims[2], sh.lbz);
+    }
     a.t2d = sw_tesla2deg(p->B0, p->timestep_us);
     a.h2d = ph->half_scale * a.t2d;
     a.t2d_f64 = (double)p->B0 * p->timestep_us * 1e-6 * kGamma * kRad2Deg;

@@ -49,6 +49,31 @@ __global__ void to_half_kernel(const float *field, uint64_t n, float inv_scale, 
     }
 }
 
+// Row-major mask/field -> bricked records (see walk_args.hpp).  KIND 0: Rec8
+// {float field_T, label}; 1: Rec4 fp16; 2: Rec4 bf16; 3: Rec4 label only.
+template <int LX, int LY, int LZ, int KIND>
+__global__ void records_kernel(const uint8_t *mask, const float *field, uint64_t n, uint32_t ny, uint32_t nz,
+                               uint64_t nb_y, uint64_t nb_z, float inv_scale, void *out) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t iz = v % nz, iy = (v / nz) % ny, ix = v / ((uint64_t)nz * ny);
+        const uint64_t b = ((ix >> LX) * nb_y + (iy >> LY)) * nb_z + (iz >> LZ);
+        const uint64_t r = (b << (LX + LY + LZ)) | ((ix & ((1u << LX) - 1)) << (LY + LZ)) |
+                           ((iy & ((1u << LY) - 1)) << LZ) | (iz & ((1u << LZ) - 1));
+        const uint32_t lab = mask[v];
+        if (KIND == 0) {
+            ((uint2 *)out)[r] = make_uint2(__float_as_uint(field ? field[v] : 0.f), lab);
+        } else {
+            uint32_t h = 0;
+            if (KIND == 1 && field) h = __half_as_ushort(__float2half_rn(field[v] * inv_scale));
+            if (KIND == 2 && field) {
+                const uint32_t bits = __float_as_uint(field[v] * inv_scale);
+                h = (bits + 0x7FFFu + ((bits >> 16) & 1u)) >> 16;      // round to nearest even
+            }
+            ((uint32_t *)out)[r] = (h & 0xFFFFu) | (lab << 16);
+        }
+    }
+}
+
 __global__ void brick_keys_kernel(const float *XYZ0, float xyz_scale, uint32_t n, double s0, double s1,
                                   double s2, int64_t nx, int64_t ny, int64_t nz, uint32_t *keys,
                                   uint32_t *vals) {
@@ -93,6 +118,32 @@ hipError_t launch_field_to_half(const float *field, uint64_t n, float inv_scale,
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(to_half_kernel, dim3((unsigned)blocks), dim3(256), 0, s, field, n, inv_scale, bf16,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t launch_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz, int kind,
+                          float inv_scale, void *out, hipStream_t s) {
+    const uint64_t n = nx * ny * nz;
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    const dim3 g((unsigned)blocks), b(256);
+    if (kind == 0) {
+        const uint64_t nby = brick_count(ny, kBrick8.lby), nbz = brick_count(nz, kBrick8.lbz);
+        hipLaunchKernelGGL((records_kernel<kBrick8.lbx, kBrick8.lby, kBrick8.lbz, 0>), g, b, 0, s, mask, field, n,
+                           (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
+    } else {
+        const uint64_t nby = brick_count(ny, kBrick4.lby), nbz = brick_count(nz, kBrick4.lbz);
+        if (kind == 1)
+            hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 1>), g, b, 0, s, mask, field, n,
+                               (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
+        else if (kind == 2)
+            hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 2>), g, b, 0, s, mask, field, n,
+                               (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
+        else
+            hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 3>), g, b, 0, s, mask, field, n,
+                               (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
+    }
     return hipGetLastError();
 }
 

@@ -124,31 +124,46 @@ struct PhaseF64 {
 // factor in float once on the host (monte_carlo.cu:243-244); multiplying the
 // gathered value by the same float factor gives the same float.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Voxel records.  The reference multiplies the fieldmap by the T->deg/step factor
+// in float once on the host (monte_carlo.cu:243-244); multiplying the stored
+// field_T by the same float factor at the gather gives the same float.
+// ---------------------------------------------------------------------------
 template <class T, bool IDX32>
-__device__ __forceinline__ T load_at(const T *base, int64_t i) {
+__device__ __forceinline__ T load_rec(const void *base, uint64_t i) {
     if (IDX32) return *(const T *)((const char *)base + (uint32_t)((uint32_t)i * (uint32_t)sizeof(T)));
-    return base[i];
+    return ((const T *)base)[i];
 }
 
-struct FieldNone {
-    template <class R, bool IDX32> __device__ static R get(const WalkArgs &, int64_t) { return (R)0; }
-};
-struct FieldF32 {
-    template <class R, bool IDX32> __device__ static R get(const WalkArgs &a, int64_t i) {
-        if (sizeof(R) == 8) return (R)((double)load_at<float, IDX32>(a.field_f32, i) * a.t2d_f64);
-        return (R)(load_at<float, IDX32>(a.field_f32, i) * a.t2d);
+struct RecF32 {                          // {float field_T, uint32 label}
+    static constexpr BrickShape shape = kBrick8;
+    using T = uint2;
+    __device__ static uint8_t label(T r) { return (uint8_t)r.y; }
+    template <class R> __device__ static R field(const WalkArgs &a, T r) {
+        if (sizeof(R) == 8) return (R)((double)__uint_as_float(r.x) * a.t2d_f64);
+        return (R)(__uint_as_float(r.x) * a.t2d);
     }
 };
-struct FieldF16 {
-    template <class R, bool IDX32> __device__ static R get(const WalkArgs &a, int64_t i) {
-        const __half h = __ushort_as_half(load_at<uint16_t, IDX32>(a.field_h, i));
-        return (R)(__half2float(h) * a.h2d);
+struct RecNone {                         // label only: {0, uint8 label, pad}
+    static constexpr BrickShape shape = kBrick4;
+    using T = uint32_t;
+    __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
+    template <class R> __device__ static R field(const WalkArgs &, T) { return (R)0; }
+};
+struct RecF16 {                          // {fp16 field_T/half_scale, uint8 label, pad}
+    static constexpr BrickShape shape = kBrick4;
+    using T = uint32_t;
+    __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
+    template <class R> __device__ static R field(const WalkArgs &a, T r) {
+        return (R)(__half2float(__ushort_as_half((unsigned short)(r & 0xFFFFu))) * a.h2d);
     }
 };
-struct FieldBF16 {
-    template <class R, bool IDX32> __device__ static R get(const WalkArgs &a, int64_t i) {
-        const float f = __uint_as_float((uint32_t)load_at<uint16_t, IDX32>(a.field_h, i) << 16);
-        return (R)(f * a.h2d);
+struct RecBF16 {                         // {bf16 field_T/half_scale, uint8 label, pad}
+    static constexpr BrickShape shape = kBrick4;
+    using T = uint32_t;
+    __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
+    template <class R> __device__ static R field(const WalkArgs &a, T r) {
+        return (R)(__uint_as_float(r << 16) * a.h2d);
     }
 };
 
@@ -205,26 +220,57 @@ struct PhiloxRng {          // production: Philox4x32-10, one block per attempte
 };
 
 // ---------------------------------------------------------------------------
-// sub2ind on truncated scaled positions (kernels.cu:140, kernels.cuh:53-60).
-// When every scaled coordinate lies in [0, n) the truncations are the
-// subscripts and the row-major index is computed in 32 bits (IDX32: nx*ny < 2^24
-// and 4*matrix_length < 2^32) with 24-bit multiplies.  Anything else takes the
+// sub2ind on truncated scaled positions (kernels.cu:140, kernels.cuh:53-60), and
+// the record of that voxel in the bricked layout.
+// When every scaled coordinate lies in [0, n) the truncations are the subscripts
+// and the arithmetic is 32-bit (IDX32: nx*ny < 2^24, every record byte offset and
+// the row-major index < 2^32) with 24-bit multiplies.  Anything else takes the
 // reference's int64 arithmetic, including its behaviour for a subscript outside
-// its axis (the index may still land inside the volume, in a neighbouring row).
+// its axis (the index may still land inside the volume, in a neighbouring row);
+// such a voxel is found again from the linear index.
 // ---------------------------------------------------------------------------
-template <bool IDX32>
-__device__ __forceinline__ int64_t voxel_index(const WalkArgs &a, const double x[3]) {
+struct Vox {
+    int64_t ind;      // row-major index (the reference's), -1 when undefined
+    uint64_t rec;     // record index in the bricked layout
+};
+
+template <class Rec, bool IDX32>
+__device__ __forceinline__ uint64_t record_of(const WalkArgs &a, uint64_t ix, uint64_t iy, uint64_t iz) {
+    constexpr int LX = Rec::shape.lbx, LY = Rec::shape.lby, LZ = Rec::shape.lbz;
+    const uint64_t inner = ((ix & ((1u << LX) - 1)) << (LY + LZ)) | ((iy & ((1u << LY) - 1)) << LZ) |
+                           (iz & ((1u << LZ) - 1));
+    if (IDX32) {
+        const uint32_t b = __umul24(__umul24((uint32_t)(ix >> LX), a.nb_y) + (uint32_t)(iy >> LY), a.nb_z) +
+                           (uint32_t)(iz >> LZ);
+        return (uint32_t)((b << (LX + LY + LZ)) | (uint32_t)inner);
+    }
+    const uint64_t b = ((ix >> LX) * a.nb_y + (iy >> LY)) * a.nb_z + (iz >> LZ);
+    return (b << (LX + LY + LZ)) | inner;
+}
+
+template <class Rec, bool IDX32>
+__device__ __forceinline__ Vox locate(const WalkArgs &a, const double x[3]) {
     const double vx = x[0] * a.s2g[0], vy = x[1] * a.s2g[1], vz = x[2] * a.s2g[2];
+    Vox v;
     if (vx >= 0.0 && vx < a.nd[0] && vy >= 0.0 && vy < a.nd[1] && vz >= 0.0 && vz < a.nd[2]) {
         const uint32_t ix = (uint32_t)vx, iy = (uint32_t)vy, iz = (uint32_t)vz;
         if (IDX32)
-            return (int64_t)(__umul24(__umul24(ix, (uint32_t)a.ny) + iy, (uint32_t)a.nz) + iz);
-        return ((int64_t)ix * a.ny + iy) * a.nz + iz;
+            v.ind = (int64_t)(__umul24(__umul24(ix, (uint32_t)a.ny) + iy, (uint32_t)a.nz) + iz);
+        else
+            v.ind = ((int64_t)ix * a.ny + iy) * a.nz + iz;
+        v.rec = record_of<Rec, IDX32>(a, ix, iy, iz);
+        return v;
     }
-    constexpr double kLim = 9.2e18;   // beyond int64: the conversion is undefined; treat as outside
-    if (!(fabs(vx) < kLim && fabs(vy) < kLim && fabs(vz) < kLim)) return -1;
+    constexpr double kLim = 9.2e18;   // beyond int64 the conversion is undefined; treat as outside
+    if (!(fabs(vx) < kLim && fabs(vy) < kLim && fabs(vz) < kLim)) { v.ind = -1; v.rec = 0; return v; }
     const int64_t ix = (int64_t)vx, iy = (int64_t)vy, iz = (int64_t)vz;
-    return ix * a.nz * a.ny + iy * a.nz + iz;
+    v.ind = ix * a.nz * a.ny + iy * a.nz + iz;
+    v.rec = 0;
+    if (v.ind >= 0 && v.ind < a.matrix_length) {
+        const uint64_t u = (uint64_t)v.ind, nz = (uint64_t)a.nz, ny = (uint64_t)a.ny;
+        v.rec = record_of<Rec, false>(a, u / (nz * ny), (u / nz) % ny, u % nz);
+    }
+    return v;
 }
 
 template <bool CROSS>
@@ -235,12 +281,6 @@ __device__ __forceinline__ double move_axis(double fov, float r, double diff, do
     return v < 0 ? v + adj : (v >= fov ? v - adj : v);
 }
 
-// Field gather at a voxel: 32-bit byte offsets from the uniform base when IDX32.
-template <class Field, class R, bool IDX32>
-__device__ __forceinline__ R gather_field(const WalkArgs &a, int64_t ind) {
-    return Field::template get<R, IDX32>(a, ind);
-}
-
 // ---------------------------------------------------------------------------
 // One spin.  Returns ST_OK / ST_LOST / ST_EXHAUSTED; counts accepted steps and
 // rejected attempts.
@@ -249,7 +289,7 @@ __device__ __forceinline__ uint32_t next_event(uint32_t a, uint32_t b, uint32_t 
     return min(min(a, b), min(c, d));
 }
 
-template <class Rng, class Field, class Phase, bool CROSS, bool IDX32>
+template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
 __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32_t &n_rej) {
     using R = typename Phase::R;
     const uint32_t spin = a.spin_offset + li;
@@ -262,12 +302,13 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
         m[i] = (R)a.M0[3 * (uint64_t)li + i];
         if (a.record_traj) xyz1[i] = x0;
     }
-    int64_t ind = voxel_index<IDX32>(a, xo);                       // kernels.cu:102
+    Vox vx = locate<Rec, IDX32>(a, xo);                            // kernels.cu:102
+    int64_t ind = vx.ind;
     if (ind < 0 || ind >= a.matrix_length) {                       // the reference reads out of bounds here
         if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
         return ST_LOST;
     }
-    uint8_t ts_old = load_at<uint8_t, IDX32>(a.mask, ind);         // :103
+    uint8_t ts_old = Rec::label(load_rec<typename Rec::T, IDX32>(a.rec, vx.rec));   // :103
     double diff = a.diffusivity[ts_old];                           // :104
     // The reference re-reads T1/T2/diffusivity of ts_old at every voxel change
     // (:167-169); they only change with ts_old, so they are read when it changes.
@@ -311,7 +352,8 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
                 xn[1] = move_axis<CROSS>(a.fov[1], rng.py, diff, xo[1]);
                 xn[2] = move_axis<CROSS>(a.fov[2], rng.pz, diff, xo[2]);
             }
-            ind = voxel_index<IDX32>(a, xn);                       // :140
+            vx = locate<Rec, IDX32>(a, xn);                        // :140
+            ind = vx.ind;
             if (ind >= a.matrix_length || ind < 0) {               // :141-147
                 if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
                 n_acc += (uint64_t)d * a.n_timepoints + tp;
@@ -320,9 +362,10 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
             const bool newvox = ind != ind_old;                    // :150
             uint8_t ts = ts_old;
             R fnew = field;
-            if (newvox) {                                          // issue the gathers first
-                ts = load_at<uint8_t, IDX32>(a.mask, ind);         // :152
-                fnew = gather_field<Field, R, IDX32>(a, ind);      // :166, read speculatively
+            if (newvox) {                                          // one gather: label and field
+                const typename Rec::T r = load_rec<typename Rec::T, IDX32>(a.rec, vx.rec);
+                ts = Rec::label(r);                                // :152
+                fnew = Rec::template field<R>(a, r);               // :166, read speculatively
             }
             if (moved) {                                           // next attempt's draws while loads fly
                 rng.commit();
@@ -424,7 +467,7 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-template <class Rng, class Field, class Phase, bool CROSS, bool IDX32>
+template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
 __global__ __launch_bounds__(256) void walk_kernel(const WalkArgs a) {
     // Blocks are dealt round-robin over the 8 XCDs; remap so that each XCD gets a
     // contiguous range of the visiting order (spatially sorted spins share L2).
@@ -436,7 +479,7 @@ __global__ __launch_bounds__(256) void walk_kernel(const WalkArgs a) {
     uint32_t n_rej = 0, lost = 0, exh = 0;
     if (t < a.n_local) {
         const uint32_t li = a.order ? a.order[t] : t;
-        const int st = walk_spin<Rng, Field, Phase, CROSS, IDX32>(a, li, n_acc, n_rej);
+        const int st = walk_spin<Rng, Rec, Phase, CROSS, IDX32>(a, li, n_acc, n_rej);
         lost = (st == ST_LOST);
         exh = (st == ST_EXHAUSTED);
     }
@@ -452,7 +495,11 @@ __global__ __launch_bounds__(256) void walk_kernel(const WalkArgs a) {
 
 template <class Rng, class F, class P>
 hipError_t launch_fp(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
-    const bool idx32 = (uint64_t)a.nx * (uint64_t)a.ny < (1ull << 24) && (uint64_t)a.matrix_length * 4 < (1ull << 32);
+    constexpr int lb = F::shape.lbx + F::shape.lby + F::shape.lbz;
+    const uint64_t records = brick_count(a.nx, F::shape.lbx) * a.nb_y * a.nb_z << lb;
+    const bool idx32 = (uint64_t)a.nx * (uint64_t)a.ny < (1ull << 24) && (uint64_t)a.matrix_length < (1ull << 32) &&
+                       records * sizeof(typename F::T) < (1ull << 32) &&
+                       (brick_count(a.nx, F::shape.lbx) * a.nb_y) < (1ull << 24) && a.nb_z < (1u << 24);
     if (a.cross_fov) {
         if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, true>), grid, block, 0, s, a);
         else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, false>), grid, block, 0, s, a);
@@ -469,18 +516,18 @@ hipError_t launch_rng(const WalkArgs &a, int field, int prec, hipStream_t s) {
     if (nb == 0) return hipSuccess;
     const dim3 block(256), grid((nb + 7) / 8 * 8);
     if (prec == 2) {
-        if (field == 0) return launch_fp<Rng, FieldNone, PhaseF64>(a, s, grid, block);
-        if (field == 1) return launch_fp<Rng, FieldF32, PhaseF64>(a, s, grid, block);
+        if (field == 0) return launch_fp<Rng, RecNone, PhaseF64>(a, s, grid, block);
+        if (field == 1) return launch_fp<Rng, RecF32, PhaseF64>(a, s, grid, block);
     } else if (prec == 1) {
-        if (field == 0) return launch_fp<Rng, FieldNone, PhaseKahan>(a, s, grid, block);
-        if (field == 1) return launch_fp<Rng, FieldF32, PhaseKahan>(a, s, grid, block);
-        if (field == 2) return launch_fp<Rng, FieldF16, PhaseKahan>(a, s, grid, block);
-        if (field == 3) return launch_fp<Rng, FieldBF16, PhaseKahan>(a, s, grid, block);
+        if (field == 0) return launch_fp<Rng, RecNone, PhaseKahan>(a, s, grid, block);
+        if (field == 1) return launch_fp<Rng, RecF32, PhaseKahan>(a, s, grid, block);
+        if (field == 2) return launch_fp<Rng, RecF16, PhaseKahan>(a, s, grid, block);
+        if (field == 3) return launch_fp<Rng, RecBF16, PhaseKahan>(a, s, grid, block);
     } else if (prec == 0) {
-        if (field == 0) return launch_fp<Rng, FieldNone, PhaseF32>(a, s, grid, block);
-        if (field == 1) return launch_fp<Rng, FieldF32, PhaseF32>(a, s, grid, block);
-        if (field == 2) return launch_fp<Rng, FieldF16, PhaseF32>(a, s, grid, block);
-        if (field == 3) return launch_fp<Rng, FieldBF16, PhaseF32>(a, s, grid, block);
+        if (field == 0) return launch_fp<Rng, RecNone, PhaseF32>(a, s, grid, block);
+        if (field == 1) return launch_fp<Rng, RecF32, PhaseF32>(a, s, grid, block);
+        if (field == 2) return launch_fp<Rng, RecF16, PhaseF32>(a, s, grid, block);
+        if (field == 3) return launch_fp<Rng, RecBF16, PhaseF32>(a, s, grid, block);
     }
     return hipErrorInvalidValue;
 }

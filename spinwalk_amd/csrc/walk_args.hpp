@@ -4,6 +4,12 @@
 #pragma once
 #include <stdint.h>
 
+#if defined(__HIPCC__)
+#define SW_HD_INLINE __host__ __device__ __forceinline__
+#else
+#define SW_HD_INLINE inline
+#endif
+
 namespace sw {
 
 // Which rotation xrot_withphase() reduces to for a given RF phase; the reference
@@ -31,11 +37,22 @@ struct Excitation {
     double  spd, cpd;
 };
 
+// Voxel records in 128-byte bricks (one cache line).  A record holds the
+// substrate label and the field of one voxel, so a voxel change is one load,
+// and a spin that moves to a neighbouring voxel usually stays in the same line.
+//   Rec8: {float field_T, uint32 label}          bricks of 4 x 2 x 2 voxels
+//   Rec4: {uint16 field bits, uint8 label, pad}   bricks of 4 x 4 x 2 voxels
+// Within a brick voxels are row-major (z fastest); bricks are row-major too.
+struct BrickShape { int lbx, lby, lbz; };
+constexpr BrickShape kBrick8 = {2, 1, 1};
+constexpr BrickShape kBrick4 = {2, 2, 1};
+
+SW_HD_INLINE uint64_t brick_count(uint64_t n, int lb) { return (n + (1ull << lb) - 1) >> lb; }
+
 struct WalkArgs {
     // ---- phantom -----------------------------------------------------------
-    const uint8_t  *mask;
-    const float    *field_f32;    // tesla per 1 T (NULL: no field)
-    const uint16_t *field_h;      // fp16/bf16 bits of field_T / half_scale
+    const void     *rec;          // bricked voxel records (Rec8 or Rec4)
+    uint32_t        nb_y, nb_z;   // bricks along y and z
     float           t2d;          // float T -> deg/step factor (monte_carlo.cu:241)
     float           h2d;          // half -> deg/step factor = half_scale * t2d
     double          t2d_f64;      // double T -> deg/step factor for the FP64 path

@@ -20,6 +20,10 @@ hipError_t launch_philox_draws(uint64_t seed, uint32_t spin0, uint32_t n_spins, 
 // uint32 bit patterns; stats must be zeroed by the caller), then half copies.
 hipError_t launch_phantom_stats(const uint8_t *mask, const float *field, uint64_t n,
                                 uint32_t *stats, hipStream_t s);
+// Row-major mask/field -> bricked voxel records: kind 0 Rec8 f32, 1 Rec4 fp16,
+// 2 Rec4 bf16, 3 Rec4 label only.  `out` must be zero-filled for padding.
+hipError_t launch_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz, int kind,
+                          float inv_scale, void *out, hipStream_t s);
 hipError_t launch_field_to_half(const float *field, uint64_t n, float inv_scale, int bf16,
                                 uint16_t *out, hipStream_t s);
 

@@ -176,11 +176,21 @@ def to_oracle(case, scale_index=0, check_transforms=True):
     return d
 
 
+def blocky_phantom(dims, n_labels=2, seed=0, block=3):
+    """Non-cubic phantom: labels constant on blocks of `block` voxels, smooth field."""
+    rng = np.random.default_rng(seed)
+    coarse = rng.integers(0, n_labels, size=[(d + block - 1) // block for d in dims])
+    mask = np.kron(coarse, np.ones((block,) * 3, np.int64))[:dims[0], :dims[1], :dims[2]].astype(np.uint8)
+    g = np.meshgrid(*[np.linspace(0, 1, d) for d in dims], indexing="ij")
+    field = (3e-8 * np.sin(7 * g[0] + 2) * np.cos(5 * g[1]) + 2e-8 * g[2] + 1e-8 * (mask == 1)).astype(np.float32)
+    return mask, field
+
+
 def make_case(cfg, mask, field_T, fov_um, n_spins=None, xyz0=None):
     from spinwalk_amd import engine as E
     if n_spins is not None:
         cfg = dict(cfg, NUMBER_OF_SPINS=n_spins)
-    fov = np.full(3, np.float32(fov_um * 1e-6), np.float32)
+    fov = (np.asarray(fov_um, np.float64) * 1e-6).astype(np.float32) * np.ones(3, np.float32)
     return E.prepare_case(cfg, mask, field_T, fov, XYZ0=xyz0)
 
 

@@ -1,0 +1,250 @@
+"""The production walk (Philox4x32-10 draws on the device).
+
+Its draws differ from the reference's, so results are pinned statistically and
+by what the mathematics fixes:
+  - the generator: Random123 known answers are checked on the host build of the
+    same code (test_philox_kat), the device draws for normal/uniform statistics;
+  - signals agree with the oracle (the reference CPU path) within 4.5 standard
+    errors of the Monte-Carlo difference;
+  - free diffusion: increments are N(0, 2 D dt) per axis (simulation_parameters.cuh:236-237);
+  - PGSE: the echo equals exp(-Var(phi)/2) with Var(phi) computed exactly for the
+    discrete gradient waveform and Gaussian steps (Stejskal-Tanner in discrete time);
+  - spin echo: |Mxy(TE)| = exp(-TE/T2) for every spin whatever its path;
+  - results do not depend on how spins are ordered or split into launches;
+  - precision modes: identical trajectories and labels; M within the stated
+    error of the FP64 path.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests import cases
+from oracle import oracle as orc
+from spinwalk_amd import engine as E
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 267515315.0
+
+
+def run(engine, case, scale_index=0, precision=E.SW_PREC_F32, field_format=E.SW_FIELD_F32, spatial_order=1,
+        spin_offset=0, n=None, formats=(E.SW_FIELD_F32,)):
+    p, xs, gs = case.scale_params(case.scales[scale_index])
+    n = case.n_spins - spin_offset if n is None else n
+    sl = slice(spin_offset, spin_offset + n)
+    ph = engine.phantom(case.mask, case.field_T, case.fov, formats=formats)
+    seq = engine.sequence(p, case.tables)
+    dt = np.float64 if precision == E.SW_PREC_F64 else np.float32
+    M1 = np.zeros((n, case.n_te, 3), dt)
+    T = np.zeros((n, case.n_te), np.uint8)
+    XYZ1 = np.zeros((n, case.traj_len, 3), np.float32)
+    st = engine.walk_host(ph, seq, p, np.ascontiguousarray(case.M0[sl]), np.ascontiguousarray(case.XYZ0[sl]), M1,
+                          XYZ1, T, spin_offset=spin_offset, xyz_scale=xs, gradient_scale=gs, precision=precision,
+                          field_format=field_format, spatial_order=spatial_order)
+    return dict(M1=M1, T=T, XYZ1=XYZ1, stats=st)
+
+
+def signal(M1, T, te, label=None):
+    sel = np.ones(T.shape[0], bool) if label is None else (T[:, te] == label)
+    v = M1[sel, te, :2].astype(np.float64)
+    mean = v.mean(axis=0)
+    s = np.linalg.norm(mean)
+    u = mean / max(s, 1e-300)
+    var = float(u @ np.cov(v.T) @ u) / max(1, len(v))       # delta method
+    return s, var, len(v)
+
+
+# ---------------------------------------------------------------------------
+def test_philox_kat():
+    """Random123 known answers for Philox4x32-10 (the host build of philox.hpp)."""
+    import ctypes, os, subprocess, tempfile
+    src = os.path.join(os.path.dirname(__file__), "..", "spinwalk_amd", "csrc", "philox.hpp")
+    code = ('#include "%s"\n#include <cstdio>\nint main(){auto a=sw::philox4x32_10(0,0,0,0,0,0);'
+            'auto b=sw::philox4x32_10(~0u,~0u,~0u,~0u,~0u,~0u);'
+            'auto c=sw::philox4x32_10(0x243f6a88,0x85a308d3,0x13198a2e,0x03707344,0xa4093822,0x299f31d0);'
+            'printf("%%08x%%08x%%08x%%08x %%08x%%08x%%08x%%08x %%08x%%08x%%08x%%08x",a.v[0],a.v[1],a.v[2],a.v[3],'
+            'b.v[0],b.v[1],b.v[2],b.v[3],c.v[0],c.v[1],c.v[2],c.v[3]);}') % os.path.abspath(src)
+    d = tempfile.mkdtemp()
+    with open(os.path.join(d, "k.cpp"), "w") as fh:
+        fh.write(code)
+    subprocess.run(["g++", "-O2", "-o", os.path.join(d, "k"), os.path.join(d, "k.cpp")], check=True)
+    out = subprocess.run([os.path.join(d, "k")], capture_output=True, text=True, check=True).stdout.split()
+    assert out == ["6627e8d5e169c58dbc57ac4c9b00dbd8", "408f276d41c83b0ea20bc7c66d5451fd",
+                   "d16cfe0994fdcceb5001e42024126ea1"]
+
+
+def test_device_draw_statistics(engine):
+    import torch
+    from scipy import stats
+    n_spins, n_att = 4096, 256
+    out = torch.zeros((n_spins, n_att, 4), dtype=torch.float32, device="cuda")
+    engine.philox_draws(10, 0, n_spins, n_att, out)
+    torch.cuda.synchronize()
+    d = out.cpu().numpy().astype(np.float64)
+    for k in range(3):
+        x = d[..., k].ravel()
+        assert abs(x.mean()) < 5 / math.sqrt(x.size)
+        assert abs(x.var() - 1) < 5 * math.sqrt(2 / x.size)
+        assert abs(stats.kurtosis(x, fisher=False) - 3) < 0.02
+        assert stats.kstest(x[:200000], "norm").pvalue > 1e-4
+    u = d[..., 3].ravel()
+    assert u.min() >= 0 and u.max() < 1
+    assert stats.kstest(u[:200000], "uniform").pvalue > 1e-4
+    lag = np.concatenate([d[:, 1:, 0], np.zeros((n_spins, 1))], axis=1).ravel()     # next attempt's x
+    c = np.corrcoef(np.stack([d[..., 0].ravel(), d[..., 1].ravel(), d[..., 2].ravel(), d[..., 3].ravel(), lag]))
+    assert np.abs(c - np.eye(5)).max() < 5 / math.sqrt(u.size)
+    # a different seed gives different streams
+    out2 = torch.zeros_like(out)
+    engine.philox_draws(11, 0, n_spins, 4, out2[:, :4].contiguous())
+    assert not torch.equal(out2[:, :4], out[:, :4])
+
+
+@pytest.mark.parametrize("kind", ["se3", "gre2_field", "bssfp"])
+def test_signals_agree_with_oracle_within_mc_error(engine, kind):
+    if kind == "se3":
+        m, f = cases.three_substrate_phantom(n=24, fov_um=48.0, seed=31)
+        cfg = cases.se(3, TE=8000, dt=50, P_XY=[[1, 0.5, 0.5], [0.5, 1, 0.5], [0.5, 0.5, 1]],
+                       T1=[2000.0, 1600.0, 1600.0], T2=[40.0, 20.0, 20.0])
+        fov = 48.0
+    elif kind == "gre2_field":
+        m, f = cases.cylinder_phantom(n=24, fov_um=24.0, radius_um=3.0, n_cyl=3, seed=32)
+        f = f * 10
+        cfg = cases.gre(2, TE=6000, dt=50)
+        fov = 24.0
+    else:
+        m, f = cases.cylinder_phantom(n=20, fov_um=40.0, radius_um=4.0, n_cyl=2, seed=33)
+        cfg = cases.bssfp(2, TE=1000, dt=50, dummies=5, lin=180.0, T1=[300.0, 250.0], T2=[50.0, 40.0],
+                          P_XY=[[1, 0.3], [0.3, 1]])
+        fov = 40.0
+    case = cases.make_case(cfg, m, f, fov, n_spins=20000)
+    g = run(engine, case)
+    o = orc.sim(cases.to_oracle(case))
+    assert g["stats"]["n_lost"] == 0 and o["n_lost"] == 0
+    labels = [None] + list(range(int(m.max()) + 1))
+    for te in range(case.n_te):
+        for lab in labels:
+            sg, vg, ng = signal(g["M1"], g["T"], te, lab)
+            so, vo, no = signal(o["M1"], o["T"], te, lab)
+            if min(ng, no) < 200:
+                continue
+            z = abs(sg - so) / math.sqrt(vg + vo + 1e-30)
+            assert z < 4.5, (kind, te, lab, sg, so, z)
+
+
+def test_free_diffusion_increments(engine):
+    mask = np.zeros((8, 8, 8), np.uint8)
+    cfg = cases.gre(1, TE=5000, dt=50, DIFFUSIVITY=[2e-9], RECORD_TRAJECTORY=1, T1=[1e9], T2=[1e9])
+    rng = np.random.default_rng(1)
+    xyz0 = rng.uniform(0.4, 0.6, size=(2000, 3)).astype(np.float32) * np.float32(1000e-6)
+    case = cases.make_case(cfg, mask, None, 1000.0, n_spins=2000, xyz0=xyz0)
+    g = run(engine, case)
+    X = g["XYZ1"].astype(np.float64)
+    inc = np.diff(X, axis=1).reshape(-1, 3)
+    sigma2 = 2 * np.float32(2e-9) * 50e-6
+    n = inc.shape[0]
+    for k in range(3):
+        assert abs(inc[:, k].mean()) < 5 * math.sqrt(sigma2 / n)
+        assert abs(inc[:, k].var() / sigma2 - 1) < 5 * math.sqrt(2 / n) + 1e-3
+    c = np.corrcoef(inc.T)
+    assert np.abs(c - np.eye(3)).max() < 5 / math.sqrt(n)
+
+
+def test_pgse_discrete_stejskal_tanner(engine):
+    D, b, delta, DELTA, dt = 2e-9, 500.0, 5.0, 10.0, 50
+    mask = np.zeros((8, 8, 8), np.uint8)
+    cfg, G = cases.pgse(1, b=b, delta_ms=delta, DELTA_ms=DELTA, dt=dt, axis=(1, 0, 0), start_ms=1.0)
+    cfg.update(DIFFUSIVITY=[D], T1=[1e9], T2=[1e9])
+    n = 60000
+    rng = np.random.default_rng(2)
+    xyz0 = rng.uniform(0.45, 0.55, size=(n, 3)).astype(np.float32) * np.float32(1000e-6)
+    case = cases.make_case(cfg, mask, None, 1000.0, n_spins=n, xyz0=xyz0)
+    g = run(engine, case)
+    # exact attenuation of the discrete sequence: phase = gamma' sum_k s_k G_k x(t_k),
+    # x(t) = x0 + sum_{j<=t} xi_j, xi ~ N(0, sigma^2); s = -1 up to the refocusing pulse
+    t_g = np.asarray(case.tables["gradient_t"])
+    Gx = np.asarray(case.tables["gx"], np.float64)
+    t_rf = int(case.tables["RF_t"][1])
+    s = np.where(t_g <= t_rf, -1.0, 1.0)
+    gp = 1e-3 * dt * 1e-6 * GAMMA
+    assert abs((s * Gx).sum()) < 1e-9 * np.abs(Gx).sum()
+    n_t = case.params.n_timepoints
+    F = np.array([(s * Gx)[t_g >= j].sum() for j in range(n_t)])
+    sigma2 = case.tables["diffusivity"][0] ** 2
+    expect = math.exp(-0.5 * gp * gp * sigma2 * float((F * F).sum()))
+    got, var, _ = signal(g["M1"], g["T"], 0)
+    assert 0.2 < expect < 0.8
+    assert abs(got - expect) < 4.5 * math.sqrt(var) + 1e-4, (got, expect)
+    # and the nominal b-value of the waveform (src/dwi/pgse.cpp:80-83) is close to it
+    assert abs(math.exp(-b * 1e6 * D) - expect) < 0.03
+
+
+def test_spin_echo_magnitude_every_spin(engine):
+    m, f = cases.cylinder_phantom(n=24, fov_um=48.0, radius_um=5.0, n_cyl=3, seed=34)
+    cfg = cases.se(2, TE=10000, dt=50, T2=[40.0, 40.0], T1=[2200.0, 2200.0], P_XY=[[1, 0.5], [0.5, 1]])
+    case = cases.make_case(cfg, m, f * 20, 48.0, n_spins=50000)
+    g = run(engine, case)
+    mag = np.linalg.norm(g["M1"][:, 0, :2].astype(np.float64), axis=1)
+    np.testing.assert_allclose(mag, math.exp(-10.0 / 40.0), rtol=2e-5)
+
+
+def test_impermeable_labels_never_change(engine):
+    m, f = cases.three_substrate_phantom(n=24, fov_um=48.0, seed=35)
+    cfg = cases.ste(3, dt=50, DIFFUSIVITY=[2e-9, 2e-9, 2e-9], MAX_ITERATIONS=1e6)
+    case = cases.make_case(cfg, m, f, 48.0, n_spins=20000)
+    g = run(engine, case)
+    idx = np.floor(case.XYZ0.astype(np.float64) * (np.asarray(m.shape) / case.fov.astype(np.float64))).astype(int)
+    lab0 = m[idx[:, 0], idx[:, 1], idx[:, 2]]
+    assert g["stats"]["n_lost"] == 0
+    for te in range(case.n_te):
+        np.testing.assert_array_equal(g["T"][:, te], lab0)
+
+
+def test_order_and_launch_split_invariance(engine):
+    m, f = cases.three_substrate_phantom(n=24, fov_um=48.0, seed=36)
+    cfg = cases.se(3, TE=6000, dt=50, P_XY=[[1, 0.5, 0.5], [0.5, 1, 0.5], [0.5, 0.5, 1]])
+    case = cases.make_case(cfg, m, f, 48.0, n_spins=10000)
+    a = run(engine, case, spatial_order=0)
+    b = run(engine, case, spatial_order=1)
+    c1 = run(engine, case, spatial_order=1, spin_offset=0, n=3333)
+    c2 = run(engine, case, spatial_order=0, spin_offset=3333, n=10000 - 3333)
+    for k in ("M1", "T", "XYZ1"):
+        np.testing.assert_array_equal(a[k], b[k])
+        np.testing.assert_array_equal(a[k], np.concatenate([c1[k], c2[k]]))
+    assert a["stats"]["n_attempts"] > a["stats"]["n_steps"]
+
+
+def test_precision_modes_against_fp64(engine):
+    m, f = cases.cylinder_phantom(n=32, fov_um=64.0, radius_um=6.0, n_cyl=4, seed=37)
+    cfg = cases.se(2, TE=40000, dt=20, P_XY=[[1, 0.5], [0.5, 1]], T2=[40.0, 20.0], T1=[2000.0, 1600.0])
+    case = cases.make_case(cfg, m, f, 64.0, n_spins=20000)
+    fm = (E.SW_FIELD_F32, E.SW_FIELD_F16, E.SW_FIELD_BF16)
+    ref = run(engine, case, precision=E.SW_PREC_F64, formats=fm)
+    res = {}
+    for name, prec, ff in (("f32", E.SW_PREC_F32, E.SW_FIELD_F32), ("kahan", E.SW_PREC_F32_KAHAN, E.SW_FIELD_F32),
+                           ("f16", E.SW_PREC_F32_KAHAN, E.SW_FIELD_F16),
+                           ("bf16", E.SW_PREC_F32_KAHAN, E.SW_FIELD_BF16)):
+        r = run(engine, case, precision=prec, field_format=ff, formats=fm)
+        np.testing.assert_array_equal(r["T"], ref["T"])
+        np.testing.assert_array_equal(r["XYZ1"], ref["XYZ1"])
+        s_ref = signal(ref["M1"], ref["T"], 0)[0]
+        s = signal(r["M1"], r["T"], 0)[0]
+        rms = float(np.sqrt(np.mean((r["M1"].astype(np.float64) - ref["M1"]) ** 2)))
+        res[name] = (abs(s - s_ref) / s_ref, rms)
+    assert res["f32"][0] < 1e-5 and res["kahan"][0] < 1e-5
+    assert res["kahan"][1] <= res["f32"][1]
+    assert res["f16"][0] < 1e-4, res
+    assert res["bf16"][0] < 2e-3, res
+
+
+def test_echo_reduce_matches_numpy(engine):
+    import torch
+    rng = np.random.default_rng(3)
+    M1 = rng.normal(size=(70000, 3, 3)).astype(np.float32)
+    T = rng.integers(0, 4, size=(70000, 3)).astype(np.uint8)
+    out = torch.zeros((3, 4, 3), dtype=torch.float64, device="cuda")
+    engine.echo_reduce(torch.from_numpy(M1).cuda(), torch.from_numpy(T).cuda(), 4, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    want = orc.echo_reduce(M1, T, 4)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-9)

@@ -167,3 +167,30 @@ def test_replay_spatial_order_is_invisible(engine):
     o = orc.sim(cases.to_oracle(case))
     g = _run_engine(engine, case, replay=rp, spatial_order=1)
     _assert_parity(g, o)
+
+
+def test_replay_odd_dims_wild_wrap_out_of_axis_subscripts(engine):
+    # non-cubic, non-multiple-of-brick dims; steps far larger than the FoV with
+    # wrap-around leave positions outside [0, fov), where the reference's int64
+    # sub2ind lands in neighbouring rows or outside the volume (kernels.cu:133-147)
+    m, f = cases.blocky_phantom((13, 11, 7), n_labels=3, seed=3)
+    cfg = cases.gre(3, TE=1000, dt=50, CROSS_FOV=1, DIFFUSIVITY=[4e-6, 3e-6, 5e-6],
+                    P_XY=[[1, 0.6, 0.4], [0.5, 1, 0.7], [0.3, 0.8, 1]])
+    case = cases.make_case(cfg, m, f, (26.0, 22.0, 14.0), n_spins=512)
+    rp = _replay_streams(case)
+    d = cases.to_oracle(case)
+    o = orc.sim(d)
+    assert 0 < o["n_lost"] < 512
+    g = _run_engine(engine, case, replay=rp)
+    _assert_parity(g, o)
+
+
+def test_replay_wide_slab_64bit_index_path(engine):
+    # nx*ny >= 2^24 selects the 64-bit index arithmetic
+    m, f = cases.blocky_phantom((4100, 4100, 1), n_labels=2, seed=4, block=5)
+    cfg = cases.gre(2, TE=500, dt=50, P_XY=[[1, 0.5], [0.5, 1]], DIFFUSIVITY=[1e-7, 1e-7])
+    case = cases.make_case(cfg, m, f, (4100.0, 4100.0, 1.0), n_spins=256)
+    rp = _replay_streams(case)
+    o = orc.sim(cases.to_oracle(case))
+    g = _run_engine(engine, case, replay=rp)
+    _assert_parity(g, o)
