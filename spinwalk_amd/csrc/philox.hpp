@@ -7,8 +7,9 @@
 //
 // One attempted step of a spin draws one Philox block keyed by the 64-bit seed
 // with counter (attempt_lo, attempt_hi, global_spin, 0).  The block gives the
-// three displacement normals (two Box-Muller pairs, the last sine dropped) and,
-// from the otherwise unused low bytes, the permeability uniform.  The stream of
+// three displacement normals (two Box-Muller pairs on the top 23 bits of the
+// words, the last sine dropped) and, from the low bytes of three words, the
+// permeability uniform.  The stream of
 // a spin depends only on (seed, spin, attempt), never on the launch shape or on
 // how spins are split over launches or devices.  As in the reference, whose
 // per-spin generators are re-seeded with seed+spin for every scale
@@ -26,11 +27,24 @@ namespace sw {
 
 struct U32x4 { uint32_t v[4]; };
 
+// a ^ b ^ c; one v_bitop3_b32 (truth table 0x96) on the device
+SW_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 SW_HD U32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                           uint32_t k0, uint32_t k1) {
     constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
     constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #if defined(__HIP_DEVICE_COMPILE__)
+    // The key is uniform across the wave.  Marking it as changed here keeps the
+    // compiler from precomputing all 20 round keys outside the caller's loop
+    // (20 SGPRs, which spill); the key schedule then runs on the scalar unit.
+    asm volatile("" : "+s"(k0), "+s"(k1));
 #pragma unroll
 #endif
     for (int r = 0; r < 10; ++r) {
@@ -38,8 +52,8 @@ SW_HD U32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
         const uint64_t p1 = (uint64_t)M1 * c2;
         const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
         const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-        const uint32_t n0 = hi1 ^ c1 ^ k0;
-        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        const uint32_t n0 = xor3(hi1, c1, k0);
+        const uint32_t n2 = xor3(hi0, c3, k1);
         c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
         k0 += W0; k1 += W1;
     }
@@ -48,9 +62,18 @@ SW_HD U32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
     return o;
 }
 
-// 24-bit uniforms from the top bits of a word.
-SW_HD float u24_open(uint32_t r)   { return ((float)(r >> 8) + 0.5f) * 5.9604644775390625e-08f; }  // (0,1)
-SW_HD float u24_closed0(uint32_t r){ return (float)(r >> 8) * 5.9604644775390625e-08f; }           // [0,1)
+// 23-bit uniforms from the top bits of a word, built as floats in [1, 2).
+SW_HD float bits_to_float(uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __uint_as_float(b);
+#else
+    float f;
+    __builtin_memcpy(&f, &b, 4);
+    return f;
+#endif
+}
+SW_HD float u23_open1(uint32_t r)  { return 2.0f - bits_to_float(0x3F800000u | (r >> 9)); }   // (0,1]
+SW_HD float u23_closed0(uint32_t r){ return bits_to_float(0x3F800000u | (r >> 9)) - 1.0f; }   // [0,1)
 // Permeability uniform from the low bytes of three words: [0,1), 24 bits.
 SW_HD float u24_lowbytes(uint32_t a, uint32_t b, uint32_t c) {
     const uint32_t x = (a & 0xFFu) | ((b & 0xFFu) << 8) | ((c & 0xFFu) << 16);

@@ -205,8 +205,8 @@ struct PhiloxRng {          // production: Philox4x32-10, one block per attempte
     __device__ void peek() {
         const U32x4 r = philox4x32_10((uint32_t)attempt, (uint32_t)(attempt >> 32), spin, 0u, k0, k1);
         // Box-Muller on (r0,r1) and (r2,r3); v_log_f32 is log2, v_sin/v_cos take revolutions
-        const float u1 = u24_open(r.v[0]), v1 = u24_closed0(r.v[1]);
-        const float u2 = u24_open(r.v[2]), v2 = u24_closed0(r.v[3]);
+        const float u1 = u23_open1(r.v[0]), v1 = u23_closed0(r.v[1]);
+        const float u2 = u23_open1(r.v[2]), v2 = u23_closed0(r.v[3]);
         const float rho1 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
         const float rho2 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u2));
         px = rho1 * __builtin_amdgcn_cosf(v1);
@@ -248,12 +248,23 @@ __device__ __forceinline__ uint64_t record_of(const WalkArgs &a, uint64_t ix, ui
     return (b << (LX + LY + LZ)) | inner;
 }
 
+// Truncating double -> int32 conversion with the instruction's defined
+// saturation (a C cast out of range would be undefined).
+__device__ __forceinline__ uint32_t cvt_i32_sat(double v) {
+    uint32_t r;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
 template <class Rec, bool IDX32>
 __device__ __forceinline__ Vox locate(const WalkArgs &a, const double x[3]) {
     const double vx = x[0] * a.s2g[0], vy = x[1] * a.s2g[1], vz = x[2] * a.s2g[2];
     Vox v;
-    if (vx >= 0.0 && vx < a.nd[0] && vy >= 0.0 && vy < a.nd[1] && vz >= 0.0 && vz < a.nd[2]) {
-        const uint32_t ix = (uint32_t)vx, iy = (uint32_t)vy, iz = (uint32_t)vz;
+    // v_cvt_i32_f64 truncates toward zero and saturates, so (unsigned)trunc(v) < n
+    // holds exactly when the reference's int64 truncation lies in [0, n).  (It
+    // maps NaN to 0; positions stay finite since every draw is finite.)
+    const uint32_t ix = cvt_i32_sat(vx), iy = cvt_i32_sat(vy), iz = cvt_i32_sat(vz);
+    if (ix < (uint32_t)a.nx && iy < (uint32_t)a.ny && iz < (uint32_t)a.nz) {
         if (IDX32)
             v.ind = (int64_t)(__umul24(__umul24(ix, (uint32_t)a.ny) + iy, (uint32_t)a.nz) + iz);
         else
@@ -263,8 +274,8 @@ __device__ __forceinline__ Vox locate(const WalkArgs &a, const double x[3]) {
     }
     constexpr double kLim = 9.2e18;   // beyond int64 the conversion is undefined; treat as outside
     if (!(fabs(vx) < kLim && fabs(vy) < kLim && fabs(vz) < kLim)) { v.ind = -1; v.rec = 0; return v; }
-    const int64_t ix = (int64_t)vx, iy = (int64_t)vy, iz = (int64_t)vz;
-    v.ind = ix * a.nz * a.ny + iy * a.nz + iz;
+    const int64_t jx = (int64_t)vx, jy = (int64_t)vy, jz = (int64_t)vz;
+    v.ind = jx * a.nz * a.ny + jy * a.nz + jz;
     v.rec = 0;
     if (v.ind >= 0 && v.ind < a.matrix_length) {
         const uint64_t u = (uint64_t)v.ind, nz = (uint64_t)a.nz, ny = (uint64_t)a.ny;

@@ -115,6 +115,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise EngineError(f"{LIB_PATH} is not built: run __graft_entry__.build() (make -C spinwalk_amd/csrc)")
+    # PyTorch-ROCm bundles its own libamdhip64.so.7 and asks for it by a name that
+    # does not match an already-loaded ROCm copy, so loading the engine first puts
+    # two HIP runtimes in the process and torch then finds no GPU.  Loading torch
+    # first makes the engine bind to torch's copy (same soname): one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     L.sw_engine_create.argtypes = [C.c_int, C.POINTER(_P)]
     L.sw_engine_destroy.argtypes = [_P]
