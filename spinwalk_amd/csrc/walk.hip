@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include <math.h>
+#include <type_traits>
 
 #include "philox.hpp"
 #include "walk_args.hpp"
@@ -229,9 +230,13 @@ struct PhiloxRng {          // production: Philox4x32-10, one block per attempte
 // its axis (the index may still land inside the volume, in a neighbouring row);
 // such a voxel is found again from the linear index.
 // ---------------------------------------------------------------------------
-struct Vox {
-    int64_t ind;      // row-major index (the reference's), -1 when undefined
-    uint64_t rec;     // record index in the bricked layout
+// A voxel lookup: ok when the reference's row-major index lies in
+// [0, matrix_length); rec is then the voxel's record index in the bricked
+// layout, a bijection of the row-major index, so records compare as voxels do.
+template <bool IDX32> using RecIdx = typename std::conditional<IDX32, uint32_t, uint64_t>::type;
+template <bool IDX32> struct Vox {
+    bool ok;
+    RecIdx<IDX32> rec;
 };
 
 template <class Rec, bool IDX32>
@@ -257,29 +262,28 @@ __device__ __forceinline__ uint32_t cvt_i32_sat(double v) {
 }
 
 template <class Rec, bool IDX32>
-__device__ __forceinline__ Vox locate(const WalkArgs &a, const double x[3]) {
+__device__ __forceinline__ Vox<IDX32> locate(const WalkArgs &a, const double x[3]) {
     const double vx = x[0] * a.s2g[0], vy = x[1] * a.s2g[1], vz = x[2] * a.s2g[2];
-    Vox v;
+    Vox<IDX32> v;
     // v_cvt_i32_f64 truncates toward zero and saturates, so (unsigned)trunc(v) < n
     // holds exactly when the reference's int64 truncation lies in [0, n).  (It
     // maps NaN to 0; positions stay finite since every draw is finite.)
     const uint32_t ix = cvt_i32_sat(vx), iy = cvt_i32_sat(vy), iz = cvt_i32_sat(vz);
     if (ix < (uint32_t)a.nx && iy < (uint32_t)a.ny && iz < (uint32_t)a.nz) {
-        if (IDX32)
-            v.ind = (int64_t)(__umul24(__umul24(ix, (uint32_t)a.ny) + iy, (uint32_t)a.nz) + iz);
-        else
-            v.ind = ((int64_t)ix * a.ny + iy) * a.nz + iz;
-        v.rec = record_of<Rec, IDX32>(a, ix, iy, iz);
+        v.ok = true;
+        v.rec = (RecIdx<IDX32>)record_of<Rec, IDX32>(a, ix, iy, iz);
         return v;
     }
-    constexpr double kLim = 9.2e18;   // beyond int64 the conversion is undefined; treat as outside
-    if (!(fabs(vx) < kLim && fabs(vy) < kLim && fabs(vz) < kLim)) { v.ind = -1; v.rec = 0; return v; }
-    const int64_t jx = (int64_t)vx, jy = (int64_t)vy, jz = (int64_t)vz;
-    v.ind = jx * a.nz * a.ny + jy * a.nz + jz;
+    v.ok = false;
     v.rec = 0;
-    if (v.ind >= 0 && v.ind < a.matrix_length) {
-        const uint64_t u = (uint64_t)v.ind, nz = (uint64_t)a.nz, ny = (uint64_t)a.ny;
-        v.rec = record_of<Rec, false>(a, u / (nz * ny), (u / nz) % ny, u % nz);
+    constexpr double kLim = 9.2e18;   // beyond int64 the conversion is undefined; treat as outside
+    if (!(fabs(vx) < kLim && fabs(vy) < kLim && fabs(vz) < kLim)) return v;
+    const int64_t jx = (int64_t)vx, jy = (int64_t)vy, jz = (int64_t)vz;
+    const int64_t ind = jx * a.nz * a.ny + jy * a.nz + jz;     // kernels.cuh:53-60
+    if (ind >= 0 && ind < a.matrix_length) {                    // kernels.cu:141
+        const uint64_t u = (uint64_t)ind, nz = (uint64_t)a.nz, ny = (uint64_t)a.ny;
+        v.ok = true;
+        v.rec = (RecIdx<IDX32>)record_of<Rec, false>(a, u / (nz * ny), (u / nz) % ny, u % nz);
     }
     return v;
 }
@@ -289,7 +293,9 @@ __device__ __forceinline__ double move_axis(double fov, float r, double diff, do
     const double rw = (double)r * diff;                          // kernels.cu:131
     const double v = xo + rw;
     const double adj = CROSS ? fov : 2 * fabs(rw);               // :133-136
-    return v < 0 ? v + adj : (v >= fov ? v - adj : v);
+    // v < 0: v + adj; v >= fov: v - adj, which rounds as v + (-adj); else v.
+    const double sadj = v < 0 ? adj : -adj;
+    return (v < 0 || v >= fov) ? v + sadj : v;
 }
 
 // ---------------------------------------------------------------------------
@@ -313,9 +319,8 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
         m[i] = (R)a.M0[3 * (uint64_t)li + i];
         if (a.record_traj) xyz1[i] = x0;
     }
-    Vox vx = locate<Rec, IDX32>(a, xo);                            // kernels.cu:102
-    int64_t ind = vx.ind;
-    if (ind < 0 || ind >= a.matrix_length) {                       // the reference reads out of bounds here
+    Vox<IDX32> vx = locate<Rec, IDX32>(a, xo);                     // kernels.cu:102
+    if (!vx.ok) {                                                  // the reference reads out of bounds here
         if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
         return ST_LOST;
     }
@@ -345,7 +350,7 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
             else
                 rot_apply<R>(e.kind, (R)a.exc_s, (R)a.exc_c, (R)e.sp, (R)e.cp, m);
         }
-        int64_t ind_old = a.matrix_length + 1;                     // :123-126
+        RecIdx<IDX32> rec_old = ~(RecIdx<IDX32>)0;                 // :123-126 (ind_old: no voxel)
         uint32_t tp = 0, old_tp = 0;
         uint32_t rf_i = 1, te_i = 0, dp_i = 0, gr_i = 0;
         uint32_t next_rf = (a.n_rf > 1) ? (uint32_t)a.rf[1].t : kNever;
@@ -364,25 +369,22 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
                 xn[2] = move_axis<CROSS>(a.fov[2], rng.pz, diff, xo[2]);
             }
             vx = locate<Rec, IDX32>(a, xn);                        // :140
-            ind = vx.ind;
-            if (ind >= a.matrix_length || ind < 0) {               // :141-147
+            if (!vx.ok) {                                          // :141-147
                 if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
                 n_acc += (uint64_t)d * a.n_timepoints + tp;
                 return ST_LOST;
             }
-            const bool newvox = ind != ind_old;                    // :150
-            uint8_t ts = ts_old;
-            R fnew = field;
-            if (newvox) {                                          // one gather: label and field
-                const typename Rec::T r = load_rec<typename Rec::T, IDX32>(a.rec, vx.rec);
-                ts = Rec::label(r);                                // :152
-                fnew = Rec::template field<R>(a, r);               // :166, read speculatively
-            }
-            if (moved) {                                           // next attempt's draws while loads fly
+            const bool newvox = vx.rec != rec_old;                 // :150
+            typename Rec::T r{};
+            if (newvox)                                            // one gather: label and field
+                r = load_rec<typename Rec::T, IDX32>(a.rec, vx.rec);
+            if (moved) {                                           // next attempt's draws while the load flies
                 rng.commit();
                 rng.peek();
             }
             if (newvox) {
+                const uint8_t ts = Rec::label(r);                  // :152
+                const R fnew = Rec::template field<R>(a, r);       // :166, read speculatively
                 if (ts != ts_old) {                                // :153-163
                     float u;
                     if (!rng.uniform(u)) { n_acc += (uint64_t)d * a.n_timepoints + tp; return ST_EXHAUSTED; }
@@ -400,7 +402,7 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
                     else                { T1 = (R)a.T1s[ts]; T2 = (R)a.T2s[ts]; }
                     diff = a.diffusivity[ts];
                 }
-                ind_old = ind;                                     // :165
+                rec_old = vx.rec;                                  // :165
                 field = fnew;
             }
             ph.add(field);                                         // :171
@@ -479,7 +481,12 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 }
 
 template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
-__global__ __launch_bounds__(256) void walk_kernel(const WalkArgs a) {
+// Occupancy target: waves per SIMD the register allocation must allow.
+#ifndef SW_WALK_WAVES_PER_EU
+#define SW_WALK_WAVES_PER_EU 6
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SW_WALK_WAVES_PER_EU)))
+void walk_kernel(const WalkArgs a) {
     // Blocks are dealt round-robin over the 8 XCDs; remap so that each XCD gets a
     // contiguous range of the visiting order (spatially sorted spins share L2).
     // The grid is a multiple of 8 blocks, so the remap is a permutation.
