@@ -224,11 +224,12 @@ struct PhiloxRng {          // production: Philox4x32-10, one block per attempte
 // sub2ind on truncated scaled positions (kernels.cu:140, kernels.cuh:53-60), and
 // the record of that voxel in the bricked layout.
 // When every scaled coordinate lies in [0, n) the truncations are the subscripts
-// and the arithmetic is 32-bit (IDX32: nx*ny < 2^24, every record byte offset and
-// the row-major index < 2^32) with 24-bit multiplies.  Anything else takes the
-// reference's int64 arithmetic, including its behaviour for a subscript outside
-// its axis (the index may still land inside the volume, in a neighbouring row);
-// such a voxel is found again from the linear index.
+// and the voxel is in range; its record index comes from per-axis tables in LDS
+// (IDX32: every record byte offset < 2^32 and nx+ny+nz tables within 64 KiB) or
+// from 64-bit arithmetic.  Anything else takes the reference's int64 arithmetic,
+// including its behaviour for a subscript outside its axis (the index may still
+// land inside the volume, in a neighbouring row); such a voxel is found again
+// from the linear index.
 // ---------------------------------------------------------------------------
 // A voxel lookup: ok when the reference's row-major index lies in
 // [0, matrix_length); rec is then the voxel's record index in the bricked
@@ -239,16 +240,38 @@ template <bool IDX32> struct Vox {
     RecIdx<IDX32> rec;
 };
 
+// The record index is separable: rec = X(ix) + Y(iy) + Z(iz) with
+//   X = (bx * nb_y * nb_z) << L | (ix mod 2^lbx) << (lby + lbz)
+//   Y = (by * nb_z) << L        | (iy mod 2^lby) << lbz
+//   Z = bz << L                 |  iz mod 2^lbz
+// (b = subscript >> lb, L = lbx + lby + lbz; the parts occupy disjoint bits, so
+// the OR is a sum).  In the 32-bit mode each block keeps the three tables in LDS:
+// axis_tab[0, nx) = X, [nx, nx+ny) = Y, [nx+ny, nx+ny+nz) = Z.
+extern __shared__ uint32_t axis_tab[];
+
+template <class Rec>
+__device__ __forceinline__ uint32_t axis_entry(const WalkArgs &a, uint32_t i) {
+    constexpr int LX = Rec::shape.lbx, LY = Rec::shape.lby, LZ = Rec::shape.lbz, L = LX + LY + LZ;
+    const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny;
+    if (i < nx)
+        return (((i >> LX) * a.nb_y * a.nb_z) << L) | ((i & ((1u << LX) - 1)) << (LY + LZ));
+    if (i < nx + ny) {
+        i -= nx;
+        return (((i >> LY) * a.nb_z) << L) | ((i & ((1u << LY) - 1)) << LZ);
+    }
+    i -= nx + ny;
+    return ((i >> LZ) << L) | (i & ((1u << LZ) - 1));
+}
+
 template <class Rec, bool IDX32>
 __device__ __forceinline__ uint64_t record_of(const WalkArgs &a, uint64_t ix, uint64_t iy, uint64_t iz) {
     constexpr int LX = Rec::shape.lbx, LY = Rec::shape.lby, LZ = Rec::shape.lbz;
+    if (IDX32) {
+        const uint32_t nx = (uint32_t)a.nx, nxy = nx + (uint32_t)a.ny;
+        return axis_tab[(uint32_t)ix] + axis_tab[nx + (uint32_t)iy] + axis_tab[nxy + (uint32_t)iz];
+    }
     const uint64_t inner = ((ix & ((1u << LX) - 1)) << (LY + LZ)) | ((iy & ((1u << LY) - 1)) << LZ) |
                            (iz & ((1u << LZ) - 1));
-    if (IDX32) {
-        const uint32_t b = __umul24(__umul24((uint32_t)(ix >> LX), a.nb_y) + (uint32_t)(iy >> LY), a.nb_z) +
-                           (uint32_t)(iz >> LZ);
-        return (uint32_t)((b << (LX + LY + LZ)) | (uint32_t)inner);
-    }
     const uint64_t b = ((ix >> LX) * a.nb_y + (iy >> LY)) * a.nb_z + (iz >> LZ);
     return (b << (LX + LY + LZ)) | inner;
 }
@@ -493,6 +516,11 @@ void walk_kernel(const WalkArgs a) {
     const uint32_t per = gridDim.x / 8, b = blockIdx.x;
     const uint32_t lb = (b % 8) * per + b / 8;
     const uint32_t t = lb * blockDim.x + threadIdx.x;
+    if (IDX32) {
+        const uint32_t n_tab = (uint32_t)(a.nx + a.ny + a.nz);
+        for (uint32_t i = threadIdx.x; i < n_tab; i += blockDim.x) axis_tab[i] = axis_entry<Rec>(a, i);
+        __syncthreads();
+    }
     uint64_t n_acc = 0;
     uint32_t n_rej = 0, lost = 0, exh = 0;
     if (t < a.n_local) {
@@ -515,14 +543,16 @@ template <class Rng, class F, class P>
 hipError_t launch_fp(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
     constexpr int lb = F::shape.lbx + F::shape.lby + F::shape.lbz;
     const uint64_t records = brick_count(a.nx, F::shape.lbx) * a.nb_y * a.nb_z << lb;
-    const bool idx32 = (uint64_t)a.nx * (uint64_t)a.ny < (1ull << 24) && (uint64_t)a.matrix_length < (1ull << 32) &&
-                       records * sizeof(typename F::T) < (1ull << 32) &&
-                       (brick_count(a.nx, F::shape.lbx) * a.nb_y) < (1ull << 24) && a.nb_z < (1u << 24);
+    // 32-bit mode: record byte offsets below 2^32 and the per-axis tables within
+    // 64 KiB of LDS.
+    const uint64_t n_tab = (uint64_t)a.nx + (uint64_t)a.ny + (uint64_t)a.nz;
+    const bool idx32 = records * sizeof(typename F::T) < (1ull << 32) && n_tab * 4 <= 65536;
+    const size_t lds = idx32 ? (size_t)n_tab * 4 : 0;
     if (a.cross_fov) {
-        if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, true>), grid, block, 0, s, a);
+        if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, true>), grid, block, lds, s, a);
         else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, false>), grid, block, 0, s, a);
     } else {
-        if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, false, true>), grid, block, 0, s, a);
+        if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, false, true>), grid, block, lds, s, a);
         else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, false, false>), grid, block, 0, s, a);
     }
     return hipGetLastError();
