@@ -340,3 +340,34 @@ def from_engine_case(case, scale_index=0, mask=None, field_T=None):
     )
     d["field_deg"] = None if field_T is None else scale_field(field_T, tesla2deg(d["B0"], dt))
     return d
+
+
+class ConfigCase:
+    """A simulation set up from a config dict with the oracle's own transforms only
+    (no engine library): the object ``from_engine_case`` reads.  Key names and
+    units are the reference's INI keys (config/config_default.ini)."""
+
+    def __init__(self, cfg, mask, field_T, fov_m, XYZ0=None, M0=None):
+        self.raw = dict(cfg)
+        self.mask = np.ascontiguousarray(mask, np.uint8)
+        self.field_T = None if field_T is None else np.ascontiguousarray(field_T, np.float32)
+        self.fov = np.asarray(fov_m, np.float32) * np.ones(3, np.float32)
+        n = int(float(cfg.get("NUMBER_OF_SPINS", 1000)))               # std::stod (config_reader.cpp:145)
+        seed = int(cfg.get("SEED", 10))
+        self.XYZ0 = init_positions(seed, self.fov, n) if XYZ0 is None else \
+            np.ascontiguousarray(XYZ0, np.float32).reshape(-1, 3)
+        self.M0 = np.tile(np.float32([0, 0, 1]), (n, 1)) if M0 is None else np.ascontiguousarray(M0, np.float32)
+        self.scales = [float(np.float32(s)) for s in cfg.get("SCALE", [1.0])]
+        self.scale_type = int(cfg.get("WHAT_TO_SCALE", 0))
+
+        class _P:
+            pass
+        self.params = _P()
+        self.params.n_spins = n
+        self.params.seed = seed
+        self.params.max_iterations = int(float(cfg.get("MAX_ITERATIONS", 1e4)))
+        self.params.enCrossFOV = int(cfg.get("CROSS_FOV", 0))
+        self.params.enRecordTrajectory = int(cfg.get("RECORD_TRAJECTORY", 0))
+
+    def oracle(self, scale_index=0):
+        return from_engine_case(self, scale_index)

@@ -120,9 +120,9 @@ def se(n_sub, TE=10000, dt=50, **kw):
                     RF_T=[0, TE // 2], TIME_STEP=dt, **kw)
 
 
-def bssfp(n_sub, TE=2500, dt=50, dummies=3, lin=180.0, **kw):
+def bssfp(n_sub, TE=2500, dt=50, dummies=3, lin=180.0, fa=16.0, **kw):
     # config_generator.cpp:147-174: TR = 2 TE, FA 16, phase cycling
-    return base_cfg(n_sub, TR=2 * TE, TE=[TE], RF_FA=[16.0], RF_PH=[0.0], RF_T=[0], TIME_STEP=dt,
+    return base_cfg(n_sub, TR=2 * TE, TE=[TE], RF_FA=[fa], RF_PH=[0.0], RF_T=[0], TIME_STEP=dt,
                     DUMMY_SCAN=dummies, LINEAR_PHASE_CYCLING=lin, **kw)
 
 
