@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--order", type=int, default=1, help="spatial visiting order (1) or spin order (0)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="wall-time budget of the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--host-io", action="store_true",
+                    help="after the timed region, time one host-buffer call (sw_walk_host: uploads, walk, downloads)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed region, run one device copy of this size (profiler calibration)")
     args = ap.parse_args()
@@ -126,6 +128,16 @@ def main():
         lost += st["n_lost"]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    if args.host_io:
+        hM1 = np.zeros(tuple(M1.shape), np.float64 if prec == 2 else np.float32)
+        hT = np.zeros(tuple(T.shape), np.uint8)
+        hX = np.zeros(tuple(XYZ1.shape), np.float32)
+        t1 = time.perf_counter()
+        hst = eng.walk_host(ph, seq, p, case.M0, case.XYZ0, hM1, hX, hT, rng=E.SW_RNG_PHILOX, precision=prec,
+                            field_format=fmt, spatial_order=args.order)
+        ht = time.perf_counter() - t1
+        log(f"[bench] host-buffer call (sw_walk_host, PCIe transfers included): {ht * 1e3:.3f} ms, "
+            f"{hst['n_steps'] / ht:.4g} spin-steps/s (kernel {hst['kernel_ms']:.3f} ms)")
     if args.copy_mib > 0:
         a = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
@@ -144,11 +156,24 @@ def main():
         with open(tpath) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
     s = sums.cpu().numpy()
+    # The same walk in FP64 (same draws, so the same trajectories): the signal error
+    # of the measured precision against it, per echo and label.
+    M1d = torch.zeros((n, n_te, 3), dtype=torch.float64, device=dev)
+    sums_d = torch.zeros_like(sums)
+    eng.walk_device(ph, seq, p, M0, XYZ0, M1d, XYZ1, T, rng=E.SW_RNG_PHILOX, precision=PRECISIONS["f64"],
+                    field_format=E.SW_FIELD_F32, spatial_order=args.order, stream=stream, stats=True)
+    eng.echo_reduce(M1d, T, p.n_substrate, sums_d, stream=stream)
+    sd = sums_d.cpu().numpy()
+    del M1d
+    errs = []
     for te in range(n_te):
         for lab in range(p.n_substrate):
-            if s[te, lab, 2] > 0:
+            if s[te, lab, 2] > 0 and sd[te, lab, 2] > 0:
                 mag = float(np.hypot(s[te, lab, 0], s[te, lab, 1]) / s[te, lab, 2])
-                log(f"[bench] echo {te} label {lab}: |<Mxy>| = {mag:.6f} over {int(s[te, lab, 2])} spins")
+                ref = float(np.hypot(sd[te, lab, 0], sd[te, lab, 1]) / sd[te, lab, 2])
+                errs.append(dict(echo=te, label=lab, signal=mag, signal_f64=ref, rel_err=abs(mag - ref) / ref))
+                log(f"[bench] echo {te} label {lab}: |<Mxy>| = {mag:.6f} over {int(s[te, lab, 2])} spins, "
+                    f"FP64 {ref:.6f}, rel. error {abs(mag - ref) / ref:.2e}")
     log(f"[bench] walk kernel {np.mean(kms):.3f} ms (min {np.min(kms):.3f}), {steps_per_launch:.4g} accepted "
         f"spin-steps/launch, lost {lost}")
     out = {
@@ -173,6 +198,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_spin_step": bpe, "kernel_ms": float(np.mean(kms))},
     }
+    out["signal_error_vs_f64"] = {"max_rel": max(e["rel_err"] for e in errs) if errs else None, "per_echo_label": errs}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"[bench] CPU baseline: oracle on {threads} threads, {args.cpu_seconds} s budget")
