@@ -62,6 +62,20 @@ SW_API int  sw_h5_read(const char *file, const char *name, int32_t dtype, void *
                        uint64_t *dims, uint32_t *ndims);
 SW_API int  sw_h5_write(const char *file, const char *name, int32_t dtype, const void *buf,
                         const uint64_t *dims, uint32_t ndims);
+/* The `config` subcommand: config_generator::generate_gre/se/bssfp and write_ini
+ * (src/config/config_generator.cpp:14-207, src/config/handler.cpp:13-39).  Writes
+ * `output` (SEQ_NAME, PHANTOM[i], the sequence's scan parameters) and, beside it,
+ * default_config.ini as its PARENT_CONFIG.  seq_name: "gre", "se" or "bssfp" in any
+ * case; SW_E_CONFIG for another name. */
+SW_API int  sw_generate_config(const char *seq_name, uint32_t TE_us, uint32_t timestep_us,
+                               const char *const *phantoms, uint32_t n_phantoms, const char *output);
+/* The `dwi` subcommand: pgse::run (src/dwi/pgse.cpp:67-153).  Rewrites config_file
+ * in place with the PGSE RF pair, the gradient waveform along dir (normalised),
+ * WHAT_TO_SCALE = 1 and SCALE[i] = sqrt(b_values[i] / b_values[0]).  The amplitude
+ * comes from b_values[0], which must be positive (a zero there divides by zero in
+ * the reference). */
+SW_API int  sw_generate_pgse(const char *config_file, const double *b_values, uint32_t n_b, const float dir[3],
+                             uint32_t start_ms, uint32_t delta_ms, uint32_t DELTA_ms);
 /* Last error message of the host library on this thread. */
 SW_API const char *sw_host_last_error(void);
 

@@ -22,6 +22,7 @@
 #include "../../../include/spinwalk_host.h"
 #include "config.hpp"
 #include "h5io.hpp"
+#include "seqgen.hpp"
 
 namespace fs = std::filesystem;
 
@@ -330,6 +331,28 @@ SW_API int sw_h5_write(const char *file, const char *name, int32_t dtype, const 
     const swhost::H5T t = dtype == 0 ? swhost::H5T::F32 : (dtype == 1 ? swhost::H5T::U8 : swhost::H5T::F64);
     std::vector<size_t> d(dims, dims + ndims);
     if (!swhost::h5_write(file, name, t, d, buf, err)) return fail(SW_E_IO, err);
+    return SW_OK;
+}
+
+// Sequence generators of the reference's `config` and `dwi` subcommands.
+SW_API int sw_generate_config(const char *seq_name, uint32_t TE_us, uint32_t timestep_us,
+                              const char *const *phantoms, uint32_t n_phantoms, const char *output) {
+    if (!seq_name || !output || (n_phantoms && !phantoms)) return fail(SW_E_INVALID, "null argument");
+    if (TE_us == 0 || timestep_us == 0) return fail(SW_E_INVALID, "TE and the time step must be positive");
+    std::vector<std::string> ph;
+    for (uint32_t i = 0; i < n_phantoms; i++) ph.emplace_back(phantoms[i] ? phantoms[i] : "");
+    std::string err;
+    if (!swhost::generate_config(seq_name, TE_us, timestep_us, ph, output, err)) return fail(SW_E_CONFIG, err);
+    return SW_OK;
+}
+
+SW_API int sw_generate_pgse(const char *config_file, const double *b_values, uint32_t n_b, const float dir[3],
+                            uint32_t start_ms, uint32_t delta_ms, uint32_t DELTA_ms) {
+    if (!config_file || !b_values || n_b == 0 || !dir) return fail(SW_E_INVALID, "null or empty argument");
+    std::string err;
+    if (!swhost::generate_pgse(config_file, std::vector<double>(b_values, b_values + n_b),
+                               std::vector<float>(dir, dir + 3), start_ms, delta_ms, DELTA_ms, err))
+        return fail(SW_E_CONFIG, err);
     return SW_OK;
 }
 

@@ -21,7 +21,8 @@ _lib = None
 
 EXPORTS = ("sw_config_load", "sw_config_destroy", "sw_config_free_string", "sw_config_params", "sw_config_scales",
            "sw_config_file", "sw_config_n_files", "sw_config_output", "sw_config_seq_name", "sw_run_config",
-           "sw_run_config_ex", "sw_sim_execute", "sw_h5_read", "sw_h5_write", "sw_host_last_error")
+           "sw_run_config_ex", "sw_sim_execute", "sw_h5_read", "sw_h5_write", "sw_generate_config",
+           "sw_generate_pgse", "sw_host_last_error")
 
 
 def load():
@@ -53,6 +54,10 @@ def load():
     L.sw_h5_read.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, _P, C.c_uint64, C.POINTER(C.c_uint64),
                              C.POINTER(C.c_uint32)]
     L.sw_h5_write.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, _P, C.POINTER(C.c_uint64), C.c_uint32]
+    L.sw_generate_config.argtypes = [C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_char_p), C.c_uint32,
+                                     C.c_char_p]
+    L.sw_generate_pgse.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_uint32, C.POINTER(C.c_float),
+                                   C.c_uint32, C.c_uint32, C.c_uint32]
     L.sw_host_last_error.restype = C.c_char_p
     _lib = L
     return L
@@ -162,3 +167,22 @@ def write_ini(path, sections):
         lines.append("")
     with open(path, "w") as fh:
         fh.write("\n".join(lines))
+
+
+def generate_config(seq_name, TE_us, timestep_us, phantoms, output):
+    """The `config` subcommand (src/config/config_generator.cpp:14-207)."""
+    L = load()
+    arr = (C.c_char_p * max(1, len(phantoms)))(*[str(x).encode() for x in phantoms])
+    if L.sw_generate_config(seq_name.encode(), int(TE_us), int(timestep_us), arr, len(phantoms),
+                            str(output).encode()) != E.SW_OK:
+        raise ConfigError(L.sw_host_last_error().decode())
+
+
+def generate_pgse(config_file, b_values, direction, start_ms, delta_ms, DELTA_ms):
+    """The `dwi` subcommand (src/dwi/pgse.cpp:67-153): rewrites config_file in place."""
+    L = load()
+    b = (C.c_double * len(b_values))(*[float(x) for x in b_values])
+    d = (C.c_float * 3)(*[float(x) for x in direction])
+    if L.sw_generate_pgse(str(config_file).encode(), b, len(b_values), d, int(start_ms), int(delta_ms),
+                          int(DELTA_ms)) != E.SW_OK:
+        raise ConfigError(L.sw_host_last_error().decode())
