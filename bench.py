@@ -91,6 +91,11 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    # A stream of our own (not the legacy default stream): torch allocates and fills
+    # on it, the engine launches on it, its HIP events time the walk on it, and
+    # torch's readbacks are ordered after.
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     eng = E.Engine(0)
     wl = W.WORKLOADS[args.workload]()
     fmt = FIELDS[args.field]
@@ -108,7 +113,6 @@ def main():
     T = torch.zeros((n, n_te), dtype=torch.uint8, device=dev)
     XYZ1 = torch.zeros((n, case.traj_len, 3), dtype=torch.float32, device=dev)
     sums = torch.zeros((n_te, p.n_substrate, 3), dtype=torch.float64, device=dev)
-    stream = torch.cuda.current_stream()
 
     def one_step():
         st = eng.walk_device(ph, seq, p, M0, XYZ0, M1, XYZ1, T, rng=E.SW_RNG_PHILOX, precision=prec,

@@ -213,6 +213,38 @@ SW_API int  sw_walk_host(sw_engine *eng, const sw_phantom *ph, const sw_sequence
                          const float *replay_normals, const float *replay_uniforms,
                          sw_walk_stats *stats);
 
+/* ---- batched walk --------------------------------------------------------------
+ * The reference walks one scale of one phantom per launch, synchronously
+ * (src/sim/monte_carlo.cu:271-342).  A batch walks many such tiles (phantoms,
+ * FoV/gradient/phase-cycling scales, sequences, spin ranges) in one launch: each
+ * tile is exactly what one sw_walk_device call with the same arguments walks, and
+ * gives the same results.  Every tile shares rng, precision and field_format (those
+ * of items[0].opts).  sw_batch_create checks every tile, builds per-tile excitation
+ * tables and visiting orders, deals the work over the 8 XCDs longest tile first,
+ * and captures counter reset, walk and the optional per-tile FP64 echo sums as one
+ * HIP graph.  The phantoms, sequences and device buffers named by the items must
+ * outlive the batch.  No counterpart in the reference: it replaces the scale loop's
+ * launches. */
+typedef struct sw_batch sw_batch;
+typedef struct sw_batch_item {
+    const sw_phantom  *phantom;
+    const sw_sequence *sequence;
+    sw_params          params;        /* the tile's parameters, as for sw_walk_device   */
+    sw_walk_opts       opts;          /* spin range, scales; replay streams per tile    */
+    const float       *d_M0, *d_XYZ0;
+    void              *d_M1;
+    float             *d_XYZ1;
+    uint8_t           *d_T;
+    double            *d_echo_sums;   /* nullable: [n_TE][n_substrate][3] FP64 sums, as
+                                         sw_echo_reduce_device writes them              */
+} sw_batch_item;
+SW_API int  sw_batch_create(sw_engine *eng, const sw_batch_item *items, uint32_t n_items, sw_batch **out);
+/* Runs the batch on `stream` (NULL: the engine's).  per_item (nullable, n_items
+ * entries) and total (nullable) are filled after a stream sync; kernel_ms is the
+ * time of the whole batch. */
+SW_API int  sw_batch_run(sw_batch *b, void *stream, sw_walk_stats *per_item, sw_walk_stats *total);
+SW_API void sw_batch_destroy(sw_batch *b);
+
 /* ---- echo reduction (demo/spinwalk_bold.ipynb:733-736), FP64 ----------------
  * out[(te*n_sub + label)*3 + {0,1,2}] = {sum Mx, sum My, count} over spins with
  * T[spin][te] == label, summed in float64 in a fixed order (deterministic). */

@@ -136,11 +136,8 @@ double exc_phase_d(float rf_phase, uint32_t d, float lin, float quad) {
     return v;
 }
 
-int ensure_exc(sw_engine *e, sw_sequence *q, const sw_params *p, hipStream_t s) {
-    if (q->exc_valid && q->key_lin == p->linear_phase_cycling && q->key_quad == p->quadratic_phase_cycling &&
-        q->key_ndummy == p->n_dummy_scan)
-        return SW_OK;
-    if (p->n_dummy_scan < 0) return fail(e, SW_E_INVALID, "n_dummy_scan < 0: call sw_params_prepare first");
+// The excitation of every dummy scan and of the measured scan (kernels.cu:110-120).
+std::vector<sw::Excitation> exc_table(const sw_sequence *q, const sw_params *p) {
     const uint32_t n = (uint32_t)p->n_dummy_scan + 1;
     std::vector<sw::Excitation> h(n);
     for (uint32_t d = 0; d < n; d++) {
@@ -153,6 +150,16 @@ int ensure_exc(sw_engine *e, sw_sequence *q, const sw_params *p, hipStream_t s) 
         h[d].spd = sin(pd * kDeg2Rad);
         h[d].cpd = cos(pd * kDeg2Rad);
     }
+    return h;
+}
+
+int ensure_exc(sw_engine *e, sw_sequence *q, const sw_params *p, hipStream_t s) {
+    if (q->exc_valid && q->key_lin == p->linear_phase_cycling && q->key_quad == p->quadratic_phase_cycling &&
+        q->key_ndummy == p->n_dummy_scan)
+        return SW_OK;
+    if (p->n_dummy_scan < 0) return fail(e, SW_E_INVALID, "n_dummy_scan < 0: call sw_params_prepare first");
+    const uint32_t n = (uint32_t)p->n_dummy_scan + 1;
+    const std::vector<sw::Excitation> h = exc_table(q, p);
     if (n > q->exc_cap) {
         SW_HIP(e, hipStreamSynchronize(s));
         dev_free(q->exc);
@@ -606,7 +613,8 @@ namespace {
 // brick holding their start position.  Neighbouring lanes then gather from
 // neighbouring voxels, and the XCD-aware block mapping of the walk kernel gives
 // each XCD (own L2) a compact region of the phantom.
-int build_order(sw_engine *e, const sw::WalkArgs &a, hipStream_t s) {
+// Writes the visiting order of a's spins into out[0, n_local).
+int build_order_into(sw_engine *e, const sw::WalkArgs &a, uint32_t *out, hipStream_t s) {
     const uint32_t n = a.n_local;
     const size_t tmp = sw::sort_pairs_temp_bytes(n);
     const size_t need = 3 * (size_t)n * sizeof(uint32_t) + tmp + 256;
@@ -617,6 +625,15 @@ int build_order(sw_engine *e, const sw::WalkArgs &a, hipStream_t s) {
         SW_HIP(e, hipMalloc(&e->d_sort, need));
         e->sort_bytes = need;
     }
+    uint32_t *kin = (uint32_t *)e->d_sort, *kout = kin + n, *vin = kout + n;
+    void *t = (void *)(((uintptr_t)(vin + n) + 255) & ~(uintptr_t)255);
+    SW_HIP(e, sw::launch_brick_keys(a.XYZ0, a.xyz_scale, n, a.s2g, a.nx, a.ny, a.nz, kin, vin, s));
+    SW_HIP(e, sw::launch_sort_pairs(t, tmp, kin, kout, vin, out, n, s));
+    return SW_OK;
+}
+
+int build_order(sw_engine *e, const sw::WalkArgs &a, hipStream_t s) {
+    const uint32_t n = a.n_local;
     if (n > e->order_cap) {
         SW_HIP(e, hipStreamSynchronize(s));
         dev_free(e->d_order);
@@ -624,22 +641,15 @@ int build_order(sw_engine *e, const sw::WalkArgs &a, hipStream_t s) {
         SW_HIP(e, hipMalloc((void **)&e->d_order, n * sizeof(uint32_t)));
         e->order_cap = n;
     }
-    uint32_t *kin = (uint32_t *)e->d_sort, *kout = kin + n, *vin = kout + n;
-    void *t = (void *)(((uintptr_t)(vin + n) + 255) & ~(uintptr_t)255);
-    SW_HIP(e, sw::launch_brick_keys(a.XYZ0, a.xyz_scale, n, a.s2g, a.nx, a.ny, a.nz, kin, vin, s));
-    SW_HIP(e, sw::launch_sort_pairs(t, tmp, kin, kout, vin, e->d_order, n, s));
-    return SW_OK;
+    return build_order_into(e, a, e->d_order, s);
 }
 
-}  // namespace
-
-SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence *cq, const sw_params *p,
-                          const sw_walk_opts *o, const float *d_M0, const float *d_XYZ0, void *d_M1,
-                          float *d_XYZ1, uint8_t *d_T, void *stream, sw_walk_stats *stats) {
-    if (!e || !ph || !cq || !p || !o) return fail(e, SW_E_INVALID, "null argument");
-    sw_sequence *q = const_cast<sw_sequence *>(cq);
-    hipStream_t s = stream ? (hipStream_t)stream : e->stream;
-    SW_HIP(e, hipSetDevice(e->device));
+// Checks one walk's arguments and fills its WalkArgs, except exc, order and
+// counters.  *field_out receives the record format the walk gathers.
+int fill_walk_args(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, const sw_params *p,
+                   const sw_walk_opts *o, const float *d_M0, const float *d_XYZ0, void *d_M1, float *d_XYZ1,
+                   uint8_t *d_T, sw::WalkArgs &a, int *field_out) {
+    if (!ph || !q || !p || !o) return fail(e, SW_E_INVALID, "null argument");
     // ---- consistency checks (the reference trusts its caller; an ABI may not)
     for (int i = 0; i < 3; i++)
         if (p->phantom_size[i] != ph->dims[i]) return fail(e, SW_E_INVALID, "phantom_size does not match phantom");
@@ -649,6 +659,7 @@ SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence 
         return fail(e, SW_E_INVALID, "mask has " + std::to_string(ph->n_labels) + " labels but only " +
                                          std::to_string(q->n_sub) + " substrates are configured");
     if (p->timestep_us <= 0) return fail(e, SW_E_INVALID, "timestep_us must be positive");
+    if (p->n_dummy_scan < 0) return fail(e, SW_E_INVALID, "n_dummy_scan < 0: call sw_params_prepare first");
     if (!(p->fov[0] > 0 && p->fov[1] > 0 && p->fov[2] > 0)) return fail(e, SW_E_INVALID, "fov must be positive");
     if ((uint64_t)o->spin_offset + o->n_local > p->n_spins)
         return fail(e, SW_E_INVALID, "spin_offset + n_local exceeds n_spins");
@@ -663,10 +674,8 @@ SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence 
     if (!ph->rec[field]) return fail(e, SW_E_INVALID, "the phantom was not created with this field format");
     if (o->precision == SW_PREC_F64 && field > SW_FIELD_F32)
         return fail(e, SW_E_INVALID, "the FP64 path gathers the float32 field");
-    int rc = ensure_exc(e, q, p, s);
-    if (rc) return rc;
+    *field_out = field;
 
-    sw::WalkArgs a;
     std::memset(&a, 0, sizeof a);
     a.rec = ph->rec[field];
     {
@@ -690,7 +699,6 @@ SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence 
     a.TE_t = q->TE_t; a.n_te = q->n_te;
     a.deph_t = q->deph_t; a.deph_deg = q->deph_deg; a.n_deph = q->n_deph;
     a.grad_t = q->grad_t; a.gx = q->gx; a.gy = q->gy; a.gz = q->gz; a.n_grad = q->n_grad;
-    a.exc = q->exc;
     a.exc_s = p->s; a.exc_c = p->c;
     a.exc_sd = sin((double)q->RF_FA0 * kDeg2Rad);
     a.exc_cd = cos((double)q->RF_FA0 * kDeg2Rad);
@@ -710,6 +718,25 @@ SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence 
     a.M0 = d_M0; a.XYZ0 = d_XYZ0; a.M1 = d_M1; a.XYZ1 = d_XYZ1; a.T = d_T;
     a.rn = o->replay_normals; a.rn_stride = o->replay_normal_stride;
     a.ru = o->replay_uniforms; a.ru_stride = o->replay_uniform_stride;
+    return SW_OK;
+}
+
+}  // namespace
+
+SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence *cq, const sw_params *p,
+                          const sw_walk_opts *o, const float *d_M0, const float *d_XYZ0, void *d_M1,
+                          float *d_XYZ1, uint8_t *d_T, void *stream, sw_walk_stats *stats) {
+    if (!e || !ph || !cq || !p || !o) return fail(e, SW_E_INVALID, "null argument");
+    sw_sequence *q = const_cast<sw_sequence *>(cq);
+    hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    SW_HIP(e, hipSetDevice(e->device));
+    sw::WalkArgs a;
+    int field = 0;
+    int rc = fill_walk_args(e, ph, q, p, o, d_M0, d_XYZ0, d_M1, d_XYZ1, d_T, a, &field);
+    if (rc) return rc;
+    rc = ensure_exc(e, q, p, s);
+    if (rc) return rc;
+    a.exc = q->exc;
     a.counters = e->d_counters;
     a.order = nullptr;
     if (o->spatial_order && o->n_local > 1) {
@@ -733,6 +760,220 @@ SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence 
     }
     return SW_OK;
 }
+
+// ---------------------------------------------------------------------------
+// batched walk
+// ---------------------------------------------------------------------------
+struct sw_batch {
+    sw_engine *eng = nullptr;
+    int device = 0;
+    uint32_t n_items = 0;
+    sw::WalkArgs *d_items = nullptr;
+    sw::BatchChunk *d_chunks = nullptr;
+    unsigned long long *d_counters = nullptr;    // [n_items][4]
+    sw::Excitation *d_exc = nullptr;             // every tile's excitation table
+    uint32_t *d_order = nullptr;                 // every tile's visiting order
+    double *d_partial = nullptr;                 // echo-reduction scratch
+    struct Group { bool cross; uint32_t offset, n_slots; };
+    std::vector<Group> groups;
+    struct Reduce { const void *M1; int f64; const uint8_t *T; uint32_t n, n_te, n_sub; double *out; };
+    std::vector<Reduce> reduces;
+    int rng = 0, field = 0, prec = 0;
+    bool idx32 = false;
+    size_t lds = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+// Counter reset, the walk of every group and the echo sums, in stream order.
+hipError_t batch_enqueue(const sw_batch *b, hipStream_t s) {
+    hipError_t err = sw::launch_zero_u64(b->d_counters, (int)(4 * b->n_items), s);
+    if (err != hipSuccess) return err;
+    for (const auto &g : b->groups) {
+        err = sw::launch_walk_batch(b->d_items, b->d_chunks + g.offset, g.n_slots, b->rng, b->field, b->prec, g.cross,
+                                    b->idx32, b->lds, s);
+        if (err != hipSuccess) return err;
+    }
+    for (const auto &r : b->reduces) {
+        err = sw::launch_echo_reduce(r.M1, r.f64, r.T, r.n, r.n_te, r.n_sub, b->d_partial, r.out, s);
+        if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+}
+
+void batch_free(sw_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (void *x : {(void *)b->d_items, (void *)b->d_chunks, (void *)b->d_counters, (void *)b->d_exc,
+                    (void *)b->d_order, (void *)b->d_partial})
+        dev_free(x);
+    delete b;
+}
+
+}  // namespace
+
+SW_API int sw_batch_create(sw_engine *e, const sw_batch_item *items, uint32_t n_items, sw_batch **out) {
+    if (!e || !out || (n_items && !items)) return fail(e, SW_E_INVALID, "null argument");
+    *out = nullptr;
+    if (n_items == 0) return fail(e, SW_E_INVALID, "empty batch");
+    SW_HIP(e, hipSetDevice(e->device));
+    hipStream_t s = e->stream;
+    std::unique_ptr<sw_batch, void (*)(sw_batch *)> b(new sw_batch, batch_free);
+    b->eng = e;
+    b->device = e->device;
+    b->n_items = n_items;
+    const sw_walk_opts &o0 = items[0].opts;
+    b->rng = o0.rng == SW_RNG_REPLAY ? 1 : 0;
+    b->prec = o0.precision;
+
+    // ---- per-tile arguments, checked as sw_walk_device checks them
+    std::vector<sw::WalkArgs> args(n_items);
+    std::vector<sw::Excitation> exc;
+    std::vector<size_t> exc_off(n_items), order_off(n_items);
+    size_t n_order = 0, partial = 0;
+    b->idx32 = true;
+    for (uint32_t i = 0; i < n_items; i++) {
+        const sw_batch_item &it = items[i];
+        if (it.opts.rng != o0.rng || it.opts.precision != o0.precision)
+            return fail(e, SW_E_INVALID, "tile " + std::to_string(i) + ": every tile of a batch shares rng and precision");
+        int field = 0;
+        int rc = fill_walk_args(e, it.phantom, it.sequence, &it.params, &it.opts, it.d_M0, it.d_XYZ0, it.d_M1,
+                                it.d_XYZ1, it.d_T, args[i], &field);
+        if (rc) { e->err = "tile " + std::to_string(i) + ": " + e->err; return rc; }
+        if (i == 0) b->field = field;
+        else if (field != b->field)
+            return fail(e, SW_E_INVALID, "tile " + std::to_string(i) + ": every tile of a batch gathers the same field format");
+        const sw::WalkMode m = sw::walk_mode(args[i], field);
+        b->idx32 = b->idx32 && m.idx32;
+        b->lds = std::max(b->lds, m.lds);
+        exc_off[i] = exc.size();
+        const std::vector<sw::Excitation> h = exc_table(it.sequence, &it.params);
+        exc.insert(exc.end(), h.begin(), h.end());
+        order_off[i] = n_order;
+        if (it.opts.spatial_order && it.opts.n_local > 1) n_order += it.opts.n_local;
+        if (it.d_echo_sums) {
+            sw_batch::Reduce r;
+            r.M1 = it.d_M1; r.f64 = it.opts.precision == SW_PREC_F64; r.T = it.d_T;
+            r.n = it.opts.n_local; r.n_te = it.sequence->n_te; r.n_sub = it.sequence->n_sub; r.out = it.d_echo_sums;
+            if (r.n_te && r.n_sub) {
+                b->reduces.push_back(r);
+                partial = std::max(partial, (size_t)sw::reduce_partial_len(r.n, r.n_te, r.n_sub));
+            }
+        }
+    }
+    if (!b->idx32) b->lds = 0;
+
+    // ---- device tables
+    SW_HIP(e, hipMalloc((void **)&b->d_counters, 4 * n_items * sizeof(unsigned long long)));
+    SW_HIP(e, dev_upload(&b->d_exc, exc.data(), exc.size(), s));
+    if (n_order) SW_HIP(e, hipMalloc((void **)&b->d_order, n_order * sizeof(uint32_t)));
+    if (partial) SW_HIP(e, hipMalloc((void **)&b->d_partial, partial * sizeof(double)));
+    for (uint32_t i = 0; i < n_items; i++) {
+        sw::WalkArgs &a = args[i];
+        a.exc = b->d_exc + exc_off[i];
+        a.counters = b->d_counters + 4 * (size_t)i;
+        a.order = nullptr;
+        if (items[i].opts.spatial_order && items[i].opts.n_local > 1) {
+            uint32_t *dst = b->d_order + order_off[i];
+            int rc = build_order_into(e, a, dst, s);
+            if (rc) return rc;
+            a.order = dst;
+        }
+    }
+    SW_HIP(e, dev_upload(&b->d_items, args.data(), args.size(), s));
+
+    // ---- chunk table: per CROSS_FOV group, each tile's 256-spin chunks split into
+    // 8 contiguous parts (one per XCD), tiles in order of decreasing steps per spin
+    std::vector<sw::BatchChunk> table;
+    for (int cross = 0; cross < 2; cross++) {
+        std::vector<uint32_t> idx;
+        for (uint32_t i = 0; i < n_items; i++)
+            if (args[i].cross_fov == cross && args[i].n_local) idx.push_back(i);
+        if (idx.empty()) continue;
+        auto cost = [&](uint32_t i) { return (uint64_t)(args[i].n_dummy + 1) * args[i].n_timepoints; };
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return cost(x) > cost(y); });
+        std::vector<sw::BatchChunk> lists[8];
+        for (uint32_t i : idx) {
+            const uint32_t nc = (args[i].n_local + 255) / 256;
+            for (uint32_t k = 0; k < 8; k++)
+                for (uint32_t c = (uint32_t)((uint64_t)nc * k / 8); c < (uint32_t)((uint64_t)nc * (k + 1) / 8); c++)
+                    lists[k].push_back({i, c * 256u});
+        }
+        size_t per = 0;
+        for (auto &l : lists) per = std::max(per, l.size());
+        sw_batch::Group g;
+        g.cross = cross != 0;
+        g.offset = (uint32_t)table.size();
+        g.n_slots = (uint32_t)(8 * per);
+        for (auto &l : lists) {
+            l.resize(per, sw::BatchChunk{sw::kNoItem, 0});
+            table.insert(table.end(), l.begin(), l.end());
+        }
+        b->groups.push_back(g);
+    }
+    SW_HIP(e, dev_upload(&b->d_chunks, table.data(), table.size(), s));
+    SW_HIP(e, hipStreamSynchronize(s));        // host tables go out of scope
+    SW_HIP(e, hipEventCreate(&b->ev0));
+    SW_HIP(e, hipEventCreate(&b->ev1));
+
+    // ---- one graph: counter reset, walks, echo sums
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const hipError_t le = batch_enqueue(b.get(), s);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(s, &g);
+        if (le == hipSuccess && ce == hipSuccess && g) {
+            b->graph = g;
+            if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) b->exec = nullptr;
+        } else if (g) {
+            (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();
+    }
+    *out = b.release();
+    return SW_OK;
+}
+
+SW_API int sw_batch_run(sw_batch *b, void *stream, sw_walk_stats *per_item, sw_walk_stats *total) {
+    if (!b) return SW_E_INVALID;
+    sw_engine *e = b->eng;
+    SW_HIP(e, hipSetDevice(b->device));
+    hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    const bool stats = per_item || total;
+    if (stats) SW_HIP(e, hipEventRecord(b->ev0, s));
+    if (b->exec) SW_HIP(e, hipGraphLaunch(b->exec, s));
+    else SW_HIP(e, batch_enqueue(b, s));
+    if (!stats) return SW_OK;
+    SW_HIP(e, hipEventRecord(b->ev1, s));
+    std::vector<unsigned long long> c(4 * (size_t)b->n_items);
+    SW_HIP(e, hipMemcpyAsync(c.data(), b->d_counters, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    SW_HIP(e, hipStreamSynchronize(s));
+    float ms = 0;
+    SW_HIP(e, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    sw_walk_stats t;
+    std::memset(&t, 0, sizeof t);
+    for (uint32_t i = 0; i < b->n_items; i++) {
+        sw_walk_stats x;
+        x.n_lost = c[4 * i]; x.n_replay_exhausted = c[4 * i + 1]; x.n_steps = c[4 * i + 2]; x.n_attempts = c[4 * i + 3];
+        x.kernel_ms = ms;
+        if (per_item) per_item[i] = x;
+        t.n_lost += x.n_lost; t.n_replay_exhausted += x.n_replay_exhausted;
+        t.n_steps += x.n_steps; t.n_attempts += x.n_attempts;
+    }
+    t.kernel_ms = ms;
+    if (total) *total = t;
+    if (t.n_replay_exhausted)
+        return fail(e, SW_E_REPLAY, std::to_string(t.n_replay_exhausted) + " replayed spins ran out of recorded draws");
+    return SW_OK;
+}
+
+SW_API void sw_batch_destroy(sw_batch *b) { batch_free(b); }
 
 SW_API int sw_walk_host(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, const sw_params *p,
                         const sw_walk_opts *o, const float *M0, const float *XYZ0, void *M1, float *XYZ1,

@@ -503,19 +503,11 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// One block of 256 lanes walks the spins at positions [t0, t0 + 256) of the
+// launch's visiting order (fewer at the end).
 template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
-// Occupancy target: waves per SIMD the register allocation must allow.
-#ifndef SW_WALK_WAVES_PER_EU
-#define SW_WALK_WAVES_PER_EU 6
-#endif
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SW_WALK_WAVES_PER_EU)))
-void walk_kernel(const WalkArgs a) {
-    // Blocks are dealt round-robin over the 8 XCDs; remap so that each XCD gets a
-    // contiguous range of the visiting order (spatially sorted spins share L2).
-    // The grid is a multiple of 8 blocks, so the remap is a permutation.
-    const uint32_t per = gridDim.x / 8, b = blockIdx.x;
-    const uint32_t lb = (b % 8) * per + b / 8;
-    const uint32_t t = lb * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void walk_block(const WalkArgs &a, uint32_t t0) {
+    const uint32_t t = t0 + threadIdx.x;
     if (IDX32) {
         const uint32_t n_tab = (uint32_t)(a.nx + a.ny + a.nz);
         for (uint32_t i = threadIdx.x; i < n_tab; i += blockDim.x) axis_tab[i] = axis_entry<Rec>(a, i);
@@ -539,15 +531,53 @@ void walk_kernel(const WalkArgs a) {
     }
 }
 
-template <class Rng, class F, class P>
-hipError_t launch_fp(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
+// Occupancy target: waves per SIMD the register allocation must allow.
+#ifndef SW_WALK_WAVES_PER_EU
+#define SW_WALK_WAVES_PER_EU 6
+#endif
+
+template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SW_WALK_WAVES_PER_EU)))
+void walk_kernel(const WalkArgs a) {
+    // Blocks are dealt round-robin over the 8 XCDs; remap so that each XCD gets a
+    // contiguous range of the visiting order (spatially sorted spins share L2).
+    // The grid is a multiple of 8 blocks, so the remap is a permutation.
+    const uint32_t per = gridDim.x / 8, b = blockIdx.x;
+    const uint32_t lb = (b % 8) * per + b / 8;
+    walk_block<Rng, Rec, Phase, CROSS, IDX32>(a, lb * blockDim.x);
+}
+
+// The batched walk: block b takes the chunk at slot (b mod 8) * (grid / 8) + b / 8
+// of the chunk table, i.e. XCD k works through slots [k * grid/8, (k+1) * grid/8)
+// in order.  The host lays each XCD's slots out longest tile first, so the long
+// walks start early and the short ones fill in at the end.
+template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SW_WALK_WAVES_PER_EU)))
+void walk_batch_kernel(const WalkArgs *__restrict__ items, const BatchChunk *__restrict__ chunks) {
+    const uint32_t per = gridDim.x / 8, b = blockIdx.x;
+    const BatchChunk c = chunks[(b % 8) * per + b / 8];
+    if (c.item == kNoItem) return;                    // padding slot (uniform across the block)
+    walk_block<Rng, Rec, Phase, CROSS, IDX32>(items[c.item], c.first);
+}
+
+// 32-bit mode: record byte offsets below 2^32 and the per-axis tables within
+// 64 KiB of LDS.
+template <class F>
+WalkMode mode_of(const WalkArgs &a) {
     constexpr int lb = F::shape.lbx + F::shape.lby + F::shape.lbz;
     const uint64_t records = brick_count(a.nx, F::shape.lbx) * a.nb_y * a.nb_z << lb;
-    // 32-bit mode: record byte offsets below 2^32 and the per-axis tables within
-    // 64 KiB of LDS.
     const uint64_t n_tab = (uint64_t)a.nx + (uint64_t)a.ny + (uint64_t)a.nz;
-    const bool idx32 = records * sizeof(typename F::T) < (1ull << 32) && n_tab * 4 <= 65536;
-    const size_t lds = idx32 ? (size_t)n_tab * 4 : 0;
+    WalkMode m;
+    m.idx32 = records * sizeof(typename F::T) < (1ull << 32) && n_tab * 4 <= 65536;
+    m.lds = m.idx32 ? (size_t)n_tab * 4 : 0;
+    return m;
+}
+
+template <class Rng, class F, class P>
+hipError_t launch_fp(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
+    const WalkMode m = mode_of<F>(a);
+    const bool idx32 = m.idx32;
+    const size_t lds = m.lds;
     if (a.cross_fov) {
         if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, true>), grid, block, lds, s, a);
         else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, false>), grid, block, 0, s, a);
@@ -556,6 +586,40 @@ hipError_t launch_fp(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
         else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, false, false>), grid, block, 0, s, a);
     }
     return hipGetLastError();
+}
+
+template <class Rng, class F, class P>
+hipError_t launch_batch_fp(const WalkArgs *items, const BatchChunk *chunks, uint32_t n_slots, bool cross,
+                           bool idx32, size_t lds, hipStream_t s) {
+    const dim3 grid(n_slots), block(256);
+    if (cross) {
+        if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, true>), grid, block, lds, s, items, chunks);
+        else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, false>), grid, block, 0, s, items, chunks);
+    } else {
+        if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, false, true>), grid, block, lds, s, items, chunks);
+        else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, false, false>), grid, block, 0, s, items, chunks);
+    }
+    return hipGetLastError();
+}
+
+template <class Rng>
+hipError_t launch_batch_rng(const WalkArgs *items, const BatchChunk *chunks, uint32_t n_slots, int field, int prec,
+                            bool cross, bool idx32, size_t lds, hipStream_t s) {
+    if (prec == 2) {
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF64>(items, chunks, n_slots, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF64>(items, chunks, n_slots, cross, idx32, lds, s);
+    } else if (prec == 1) {
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseKahan>(items, chunks, n_slots, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseKahan>(items, chunks, n_slots, cross, idx32, lds, s);
+        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseKahan>(items, chunks, n_slots, cross, idx32, lds, s);
+        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseKahan>(items, chunks, n_slots, cross, idx32, lds, s);
+    } else if (prec == 0) {
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF32>(items, chunks, n_slots, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF32>(items, chunks, n_slots, cross, idx32, lds, s);
+        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseF32>(items, chunks, n_slots, cross, idx32, lds, s);
+        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseF32>(items, chunks, n_slots, cross, idx32, lds, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 template <class Rng>
@@ -585,6 +649,24 @@ hipError_t launch_rng(const WalkArgs &a, int field, int prec, hipStream_t s) {
 hipError_t launch_walk(const WalkArgs &a, int rng, int field, int prec, hipStream_t s) {
     if (rng == 1) return launch_rng<ReplayRng>(a, field, prec, s);
     if (rng == 0) return launch_rng<PhiloxRng>(a, field, prec, s);
+    return hipErrorInvalidValue;
+}
+
+WalkMode walk_mode(const WalkArgs &a, int field) {
+    switch (field) {
+    case 0: return mode_of<RecNone>(a);
+    case 1: return mode_of<RecF32>(a);
+    case 2: return mode_of<RecF16>(a);
+    default: return mode_of<RecBF16>(a);
+    }
+}
+
+hipError_t launch_walk_batch(const WalkArgs *items, const BatchChunk *chunks, uint32_t n_slots, int rng, int field,
+                             int prec, bool cross, bool idx32, size_t lds, hipStream_t s) {
+    if (n_slots == 0) return hipSuccess;
+    if (n_slots % 8) return hipErrorInvalidValue;
+    if (rng == 1) return launch_batch_rng<ReplayRng>(items, chunks, n_slots, field, prec, cross, idx32, lds, s);
+    if (rng == 0) return launch_batch_rng<PhiloxRng>(items, chunks, n_slots, field, prec, cross, idx32, lds, s);
     return hipErrorInvalidValue;
 }
 

@@ -101,4 +101,9 @@ struct WalkArgs {
     unsigned long long *counters;
 };
 
+// A work unit of the batched walk: positions [first, first + 256) of one item's
+// visiting order.  Padding slots carry item == kNoItem.
+struct BatchChunk { uint32_t item, first; };
+constexpr uint32_t kNoItem = 0xFFFFFFFFu;
+
 }  // namespace sw

@@ -71,6 +71,13 @@ class SwWalkOpts(C.Structure):
     ]
 
 
+class SwBatchItem(C.Structure):
+    _fields_ = [
+        ("phantom", _P), ("sequence", _P), ("params", SwParams), ("opts", SwWalkOpts),
+        ("d_M0", _P), ("d_XYZ0", _P), ("d_M1", _P), ("d_XYZ1", _P), ("d_T", _P), ("d_echo_sums", _P),
+    ]
+
+
 class SwShapeSpec(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("label", C.c_uint32), ("radius_um", C.c_float), ("volume_fraction", C.c_float),
@@ -98,6 +105,7 @@ EXPORTS = (
     "sw_version", "sw_params_prepare", "sw_timing_scale", "sw_tesla2deg", "sw_init_positions",
     "sw_replay_fill", "sw_phantom_create", "sw_phantom_create_device", "sw_phantom_destroy",
     "sw_phantom_info", "sw_phantom_download", "sw_phantom_generate", "sw_sequence_create", "sw_sequence_destroy", "sw_walk_device", "sw_walk_host",
+    "sw_batch_create", "sw_batch_run", "sw_batch_destroy",
     "sw_echo_reduce_device", "sw_philox_draws_device", "sw_stream_sync",
 )
 
@@ -157,8 +165,25 @@ def load():
     L.sw_echo_reduce_device.argtypes = [_P, _P, C.c_int, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]
     L.sw_philox_draws_device.argtypes = [_P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]
     L.sw_stream_sync.argtypes = [_P, _P]
+    L.sw_batch_create.argtypes = [_P, C.POINTER(SwBatchItem), C.c_uint32, C.POINTER(_P)]
+    L.sw_batch_run.argtypes = [_P, _P, C.POINTER(SwWalkStats), C.POINTER(SwWalkStats)]
+    L.sw_batch_destroy.argtypes = [_P]
+    L.sw_batch_destroy.restype = None
     _lib = L
     return L
+
+
+def _stream_arg(stream):
+    """(hipStream_t handle or None, sync_after) for a torch stream argument.
+
+    None selects the engine's own stream (the caller synchronises, e.g. with
+    torch.cuda.synchronize()).  A NULL handle at the C ABI also means the engine's
+    stream, so torch's legacy default stream (handle 0) cannot be named there: work
+    given that stream runs on the engine's stream and the call waits for it."""
+    if stream is None:
+        return None, False
+    h = int(stream.cuda_stream)
+    return (None, True) if h == 0 else (_P(h), False)
 
 
 def _np_ptr(a):
@@ -467,29 +492,96 @@ class Engine:
             o.replay_normals, o.replay_uniforms = replay[0].data_ptr(), replay[1].data_ptr()
             o.replay_normal_stride, o.replay_uniform_stride = replay[0].shape[1], replay[1].shape[1]
         st = SwWalkStats()
-        sp = None if stream is None else _P(stream.cuda_stream)
+        sp, wait = _stream_arg(stream)
         ptr = lambda t: None if t is None else _P(t.data_ptr())
         rc = self.L.sw_walk_device(self.h, ph.h, seq.h, C.byref(params), C.byref(o), ptr(M0), ptr(XYZ0),
                                    ptr(M1), ptr(XYZ1), ptr(T), sp, C.byref(st) if stats else None)
         self._check(rc, "sw_walk_device")
+        if wait:
+            self.sync()
         return st.as_dict() if stats else None
+
+    def batch(self, items):
+        """A Batch of tiles; each item is a dict with phantom, sequence, params and
+        device tensors M0, XYZ0, M1, XYZ1, T (optional: spin_offset, n_local,
+        xyz_scale, gradient_scale, rng, precision, field_format, spatial_order,
+        replay=(normals, uniforms), echo_sums)."""
+        return Batch(self, items)
 
     def echo_reduce(self, d_M1, d_T, n_sub, d_out, stream=None):
         import torch
         n_spins, n_te = d_T.shape[0], d_T.shape[1]
         dt = SW_DTYPE_F64 if d_M1.dtype == torch.float64 else SW_DTYPE_F32
-        sp = None if stream is None else _P(stream.cuda_stream)
+        sp, wait = _stream_arg(stream)
         self._check(self.L.sw_echo_reduce_device(self.h, _P(d_M1.data_ptr()), dt, _P(d_T.data_ptr()), n_spins, n_te,
                                                  n_sub, _P(d_out.data_ptr()), sp), "sw_echo_reduce_device")
+        if wait:
+            self.sync()
 
     def philox_draws(self, seed, spin0, n_spins, n_att, d_out, stream=None):
-        sp = None if stream is None else _P(stream.cuda_stream)
+        sp, wait = _stream_arg(stream)
         self._check(self.L.sw_philox_draws_device(self.h, int(seed), int(spin0), int(n_spins), int(n_att),
                                                   _P(d_out.data_ptr()), sp), "sw_philox_draws_device")
+        if wait:
+            self.sync()
 
     def sync(self, stream=None):
-        sp = None if stream is None else _P(stream.cuda_stream)
+        sp, _ = _stream_arg(stream)
         self._check(self.L.sw_stream_sync(self.h, sp), "sw_stream_sync")
+
+
+class Batch:
+    """Many tiles walked in one launch (sw_batch_create / sw_batch_run).  Keeps the
+    tiles' tensors, phantoms and sequences alive while it lives."""
+
+    def __init__(self, eng, items):
+        self.eng = eng
+        self.keep = list(items)
+        arr = (SwBatchItem * len(items))()
+        ptr = lambda t: None if t is None else _P(t.data_ptr())
+        for i, it in enumerate(items):
+            b = arr[i]
+            b.phantom, b.sequence = it["phantom"].h, it["sequence"].h
+            C.pointer(b.params)[0] = it["params"]
+            o = b.opts
+            o.spin_offset = int(it.get("spin_offset", 0))
+            o.n_local = int(it.get("n_local", it["M0"].shape[0]))
+            o.xyz_scale, o.gradient_scale = float(it.get("xyz_scale", 1.0)), float(it.get("gradient_scale", 1.0))
+            o.rng, o.precision = int(it.get("rng", SW_RNG_PHILOX)), int(it.get("precision", SW_PREC_F32))
+            o.field_format, o.spatial_order = int(it.get("field_format", SW_FIELD_F32)), int(it.get("spatial_order", 1))
+            rp = it.get("replay")
+            if rp is not None:
+                o.replay_normals, o.replay_uniforms = rp[0].data_ptr(), rp[1].data_ptr()
+                o.replay_normal_stride, o.replay_uniform_stride = rp[0].shape[1], rp[1].shape[1]
+            b.d_M0, b.d_XYZ0, b.d_M1 = ptr(it["M0"]), ptr(it["XYZ0"]), ptr(it["M1"])
+            b.d_XYZ1, b.d_T, b.d_echo_sums = ptr(it["XYZ1"]), ptr(it["T"]), ptr(it.get("echo_sums"))
+        h = _P()
+        eng._check(eng.L.sw_batch_create(eng.h, arr, len(items), C.byref(h)), "sw_batch_create")
+        self.h = h
+        self.n = len(items)
+
+    def run(self, stream=None, stats=True):
+        sp, wait = _stream_arg(stream)
+        if not stats:
+            self.eng._check(self.eng.L.sw_batch_run(self.h, sp, None, None), "sw_batch_run")
+            if wait:
+                self.eng.sync()
+            return None
+        per = (SwWalkStats * self.n)()
+        tot = SwWalkStats()
+        self.eng._check(self.eng.L.sw_batch_run(self.h, sp, per, C.byref(tot)), "sw_batch_run")
+        return dict(total=tot.as_dict(), items=[per[i].as_dict() for i in range(self.n)])
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.eng.L.sw_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Phantom:

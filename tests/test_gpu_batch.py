@@ -1,0 +1,147 @@
+"""The batched walk (sw_batch_*): many tiles in one launch, captured as a HIP graph.
+
+A tile of a batch is exactly what one sw_walk_device call walks with the same
+arguments, and its draws depend only on (seed, global spin, attempt) or on its own
+replay streams, so every tile must reproduce the separate call bit for bit:
+trajectories, labels, magnetisation, counters and FP64 echo sums.  The graph is
+replayed several times with eager work of the program in between, and every
+replay is checked against the separate calls.
+"""
+import numpy as np
+import pytest
+
+from tests import cases
+from spinwalk_amd import engine as E
+
+pytestmark = pytest.mark.gpu
+
+
+def _tile(engine, torch, case, scale_index=0, *, spin_offset=0, n=None, precision=E.SW_PREC_F32,
+          field_format=E.SW_FIELD_F32, rng=E.SW_RNG_PHILOX, formats=(E.SW_FIELD_F32,)):
+    p, xs, gs = case.scale_params(case.scales[scale_index])
+    n = case.n_spins - spin_offset if n is None else n
+    sl = slice(spin_offset, spin_offset + n)
+    ph = engine.phantom(case.mask, case.field_T, case.fov, formats=formats)
+    seq = engine.sequence(p, case.tables)
+    dt = torch.float64 if precision == E.SW_PREC_F64 else torch.float32
+    t = dict(phantom=ph, sequence=seq, params=p, spin_offset=spin_offset, n_local=n, xyz_scale=xs,
+             gradient_scale=gs, precision=precision, field_format=field_format, rng=rng, spatial_order=1,
+             M0=torch.from_numpy(np.ascontiguousarray(case.M0[sl])).cuda(),
+             XYZ0=torch.from_numpy(np.ascontiguousarray(case.XYZ0[sl])).cuda(),
+             M1=torch.zeros((n, case.n_te, 3), dtype=dt, device="cuda"),
+             T=torch.zeros((n, case.n_te), dtype=torch.uint8, device="cuda"),
+             XYZ1=torch.zeros((n, case.traj_len, 3), dtype=torch.float32, device="cuda"),
+             echo_sums=torch.zeros((case.n_te, p.n_substrate, 3), dtype=torch.float64, device="cuda"))
+    if rng == E.SW_RNG_REPLAY:
+        steps = case.steps_per_spin()
+        nn, uu = E.replay_fill(int(p.seed), spin_offset, n, 9 * steps + 96, 3 * steps + 64)
+        t["replay"] = (torch.from_numpy(nn).cuda(), torch.from_numpy(uu).cuda())
+    return t
+
+
+def _separate(engine, torch, t):
+    """The same tile through sw_walk_device, into fresh buffers."""
+    M1, T, X = torch.zeros_like(t["M1"]), torch.zeros_like(t["T"]), torch.zeros_like(t["XYZ1"])
+    st = engine.walk_device(t["phantom"], t["sequence"], t["params"], t["M0"], t["XYZ0"], M1, X, T,
+                            spin_offset=t["spin_offset"], n_local=t["n_local"], xyz_scale=t["xyz_scale"],
+                            gradient_scale=t["gradient_scale"], rng=t["rng"], precision=t["precision"],
+                            field_format=t["field_format"], spatial_order=0, replay=t.get("replay"))
+    sums = torch.zeros_like(t["echo_sums"])
+    engine.echo_reduce(M1, T, t["params"].n_substrate, sums)
+    torch.cuda.synchronize()
+    return dict(M1=M1.cpu().numpy(), T=T.cpu().numpy(), XYZ1=X.cpu().numpy(), sums=sums.cpu().numpy(), stats=st)
+
+
+def _check(t, ref, st=None):
+    np.testing.assert_array_equal(t["XYZ1"].cpu().numpy(), ref["XYZ1"])
+    np.testing.assert_array_equal(t["T"].cpu().numpy(), ref["T"])
+    np.testing.assert_array_equal(t["M1"].cpu().numpy(), ref["M1"])
+    np.testing.assert_array_equal(t["echo_sums"].cpu().numpy(), ref["sums"])
+    if st is not None:
+        for k in ("n_lost", "n_steps", "n_attempts"):
+            assert st[k] == ref["stats"][k], k
+
+
+def _clear(tiles):
+    for t in tiles:
+        t["M1"].zero_(); t["T"].zero_(); t["XYZ1"].zero_(); t["echo_sums"].zero_()
+
+
+def _cases():
+    m1, f1 = cases.cylinder_phantom(n=24, fov_um=48.0, radius_um=5.0, n_cyl=3, seed=41)
+    m2, f2 = cases.three_substrate_phantom(n=24, fov_um=48.0, seed=42)
+    m3, f3 = cases.blocky_phantom((17, 9, 31), n_labels=2, seed=43)
+    gre = cases.gre(2, TE=4000, dt=50, P_XY=[[1, 0.5], [0.5, 1]], SCALE=[1.0, 1.7])
+    se = cases.se(3, TE=6000, dt=50, CROSS_FOV=1, P_XY=[[1, 0.3, 0.3], [0.3, 1, 0.3], [0.3, 0.3, 1]])
+    ssfp = cases.bssfp(2, TE=1000, dt=50, dummies=4, lin=90.0, WHAT_TO_SCALE=2, SCALE=[1.0, 0.5],
+                       P_XY=[[1, 0.5], [0.5, 1]])
+    pg, _ = cases.pgse(2, b=800.0, delta_ms=1.0, DELTA_ms=2.0, dt=50, axis=(0.3, 1.0, 0.2), start_ms=0.5,
+                       P_XY=[[1, 0.5], [0.5, 1]], WHAT_TO_SCALE=1, SCALE=[1.0, 0.7])
+    return [
+        (cases.make_case(gre, m1, f1 * 30, 48.0, n_spins=3000), 0, {}),
+        (cases.make_case(gre, m1, f1 * 30, 48.0, n_spins=3000), 1, {}),               # FoV scale
+        (cases.make_case(se, m2, f2, 48.0, n_spins=2000), 0, dict(spin_offset=300, n=1300)),
+        (cases.make_case(ssfp, m3, f3, (17.0, 9.0, 31.0), n_spins=1500), 1, {}),      # phase-cycling scale
+        (cases.make_case(pg, m1, f1, 48.0, n_spins=2500), 1, {}),                     # gradient scale
+        (cases.make_case(gre, m1, None, 48.0, n_spins=700), 0, {}),                   # no fieldmap
+    ]
+
+
+def test_batch_equals_separate_walks_over_graph_replays(engine):
+    import torch
+    cs = _cases()
+    # the no-field tile gathers label-only records: a batch shares one record
+    # format, so it gets a batch of its own
+    tiles = [_tile(engine, torch, c, si, **kw) for c, si, kw in cs[:-1]]
+    refs = [_separate(engine, torch, t) for t in tiles]
+    b = engine.batch(tiles)
+    other = _tile(engine, torch, cs[0][0])
+    for rep in range(3):
+        _clear(tiles)
+        st = b.run()
+        torch.cuda.synchronize()
+        for t, r, s in zip(tiles, refs, st["items"]):
+            _check(t, r, s)
+        assert st["total"]["n_steps"] == sum(r["stats"]["n_steps"] for r in refs)
+        # eager work of the program between replays
+        _separate(engine, torch, other)
+        torch.cuda.synchronize()
+    b.close()
+    nf = [_tile(engine, torch, cs[-1][0])]
+    ref = _separate(engine, torch, nf[0])
+    b2 = engine.batch(nf)
+    b2.run()
+    torch.cuda.synchronize()
+    _check(nf[0], ref)
+
+
+def test_batch_replay_mode_and_precisions(engine):
+    import torch
+    cs = _cases()
+    for prec, ff, rng in ((E.SW_PREC_F32, E.SW_FIELD_F32, E.SW_RNG_REPLAY),
+                          (E.SW_PREC_F64, E.SW_FIELD_F32, E.SW_RNG_PHILOX),
+                          (E.SW_PREC_F32_KAHAN, E.SW_FIELD_F16, E.SW_RNG_PHILOX)):
+        fm = (E.SW_FIELD_F32, E.SW_FIELD_F16)
+        tiles = [_tile(engine, torch, c, si, precision=prec, field_format=ff, rng=rng, formats=fm, **kw)
+                 for c, si, kw in cs[:3]]
+        refs = [_separate(engine, torch, t) for t in tiles]
+        b = engine.batch(tiles)
+        for _ in range(2):
+            _clear(tiles)
+            st = b.run()
+            torch.cuda.synchronize()
+            for t, r, s in zip(tiles, refs, st["items"]):
+                _check(t, r, s)
+        b.close()
+
+
+def test_batch_rejects_mixed_modes(engine):
+    import torch
+    cs = _cases()
+    a = _tile(engine, torch, cs[0][0])
+    c = _tile(engine, torch, cs[0][0], precision=E.SW_PREC_F32_KAHAN)
+    with pytest.raises(E.EngineError, match="shares rng and precision"):
+        engine.batch([a, c])
+    d = _tile(engine, torch, cs[-1][0])
+    with pytest.raises(E.EngineError, match="same field format"):
+        engine.batch([a, d])
