@@ -63,6 +63,116 @@ def cpu_baseline(case, ph, budget_s, threads):
                 walk_only_value=walk_only)
 
 
+def _signals(sums):
+    """|<Mxy>| per (echo, label) from FP64 echo sums, None where no spin counts."""
+    out = {}
+    for te in range(sums.shape[0]):
+        for lab in range(sums.shape[1]):
+            if sums[te, lab, 2] > 0:
+                out[(te, lab)] = float(np.hypot(sums[te, lab, 0], sums[te, lab, 1]) / sums[te, lab, 2])
+    return out
+
+
+def _max_rel(a, ref):
+    errs = [abs(a[k] - ref[k]) / ref[k] for k in ref if k in a and ref[k] > 0]
+    return max(errs) if errs else None
+
+
+def main_tiles(args, torch, E, W, eng, stream, dev):
+    """A workload of many tiles (the FoV-scaling sweep): the reference's way, one
+    plain FP32 walk per tile in turn, against one batched launch in the requested
+    precision and field format, each with the FP64 echo sums per tile; the signal
+    error of both against the FP64 walk of the same draws."""
+    wl = W.WORKLOADS[args.workload]()
+    fmt, prec = FIELDS[args.field], PRECISIONS[args.precision]
+    log(f"[bench] workload {wl['name']}: generating phantom on the device")
+    ph, cases_, vf = W.build_tiles(eng, wl, formats=(E.SW_FIELD_F32, fmt))
+    tiles = []
+    for case in cases_:
+        p, xs, gs = case.scale_params(case.scales[0])
+        n = case.n_spins
+        z = lambda *sh, dt=torch.float32: torch.zeros(sh, dtype=dt, device=dev)
+        tiles.append(dict(phantom=ph, sequence=eng.sequence(p, case.tables), params=p, xyz_scale=xs,
+                          gradient_scale=gs, spin_offset=0, n_local=n, spatial_order=args.order,
+                          M0=torch.from_numpy(case.M0).to(dev), XYZ0=torch.from_numpy(case.XYZ0).to(dev),
+                          M1=z(n, case.n_te, 3), T=z(n, case.n_te, dt=torch.uint8), XYZ1=z(n, case.traj_len, 3),
+                          echo_sums=z(case.n_te, p.n_substrate, 3, dt=torch.float64), steps=case.steps_per_spin()))
+    total_steps = sum(t["n_local"] * t["steps"] for t in tiles)
+
+    def sequential():
+        for t in tiles:
+            eng.walk_device(t["phantom"], t["sequence"], t["params"], t["M0"], t["XYZ0"], t["M1"], t["XYZ1"], t["T"],
+                            xyz_scale=t["xyz_scale"], gradient_scale=t["gradient_scale"], precision=0,
+                            field_format=E.SW_FIELD_F32, spatial_order=args.order, stream=stream, stats=False)
+            eng.echo_reduce(t["M1"], t["T"], t["params"].n_substrate, t["echo_sums"], stream=stream)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps
+
+    seq_s = timed(sequential)
+    seq_sig = [_signals(t["echo_sums"].cpu().numpy()) for t in tiles]
+    log(f"[bench] one plain FP32 walk per tile: {seq_s * 1e3:.2f} ms, {total_steps / seq_s:.4g} spin-steps/s")
+
+    def make_batch(precision, field):
+        dt = torch.float64 if precision == 2 else torch.float32
+        items = []
+        for t in tiles:
+            it = dict(t)
+            it.update(precision=precision, field_format=field, M1=torch.zeros(tuple(t["M1"].shape), dtype=dt, device=dev),
+                      echo_sums=torch.zeros_like(t["echo_sums"]))
+            items.append(it)
+        return eng.batch(items), items
+
+    b, items = make_batch(prec, fmt)
+    st = {}
+
+    def batched():
+        st.update(b.run(stream=stream, stats=True)["total"])
+
+    b_s = timed(batched)
+    b_sig = [_signals(it["echo_sums"].cpu().numpy()) for it in items]
+    log(f"[bench] batched ({args.precision}, {args.field} field): {b_s * 1e3:.2f} ms, "
+        f"{total_steps / b_s:.4g} spin-steps/s, graph time {st['kernel_ms']:.2f} ms, lost {st['n_lost']}")
+    b64, items64 = make_batch(2, E.SW_FIELD_F32)
+    b64.run(stream=stream, stats=True)
+    ref_sig = [_signals(it["echo_sums"].cpu().numpy()) for it in items64]
+    err_b = max((_max_rel(a, r) or 0.0) for a, r in zip(b_sig, ref_sig))
+    err_s = max((_max_rel(a, r) or 0.0) for a, r in zip(seq_sig, ref_sig))
+    log(f"[bench] max relative signal error against FP64: batched {err_b:.2e}, plain FP32 {err_s:.2e}")
+    bpe = 5 if fmt == E.SW_FIELD_F32 else 3
+    achieved = total_steps * bpe / (st["kernel_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": "spin-steps/s", "value": total_steps / b_s, "unit": "spin-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": b_s * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": {0: "f32", 1: "f32-kahan", 2: "f64"}[prec] + ("" if fmt == 1 else f"+{args.field}-field"),
+        "data": "synthetic",
+        "config": {"workload": wl["name"], "phantom": "256^3 voxels, 256 um, random cylinders r8 4%",
+                   "tiles": len(tiles), "spins_per_tile": tiles[0]["n_local"], "spin_steps": total_steps,
+                   "precision": args.precision, "field": args.field, "mode": "batched, one HIP graph",
+                   "rng": "philox4x32-10"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_spin_step": bpe,
+                     "kernel_ms": st["kernel_ms"]},
+        "sequential_f32": {"value": total_steps / seq_s, "ms_per_step": seq_s * 1e3,
+                           "signal_error_vs_f64": err_s},
+        "signal_error_vs_f64": {"max_rel": err_b},
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"[bench] CPU baseline: oracle on {threads} threads, {args.cpu_seconds} s budget, first tile")
+        out["cpu_baseline"] = cpu_baseline(cases_[0], ph, args.cpu_seconds, threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +207,8 @@ def main():
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
     eng = E.Engine(0)
+    if args.workload == "c4":
+        return main_tiles(args, torch, E, W, eng, stream, dev)
     wl = W.WORKLOADS[args.workload]()
     fmt = FIELDS[args.field]
     prec = PRECISIONS[args.precision]

@@ -9,6 +9,8 @@ TR = 2 TE with FA 16.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 from . import engine as E
@@ -56,7 +58,42 @@ def c2():
     return dict(name="c2_se_500cube_1e6", phantom=phantom, cfg=cfg)
 
 
-WORKLOADS = {"c1": c1, "c2": c2}
+def _divisor_at_most(n, limit, lowest):
+    """The largest divisor of n in [lowest, limit] (lowest if none is)."""
+    best = lowest
+    for d in range(lowest, max(lowest, limit) + 1):
+        if n % d == 0:
+            best = d
+    return best
+
+
+def c4():
+    """FoV-scaling sweep: one 256^3 phantom of random cylinders (r 8 um in a
+    256 um FoV, 4 %) walked at 32 FoV scales s, log-spaced so that the scaled
+    radius 8 s um spans 1-100 um (WHAT_TO_SCALE = 0), each with GRE TE 20 ms and SE
+    TE 40 ms and 1e5 spins: 64 tiles.  Each tile has its own time step: the largest
+    divisor of TE (so echo and refocusing fall on the grid) with
+    sqrt(2 D dt) <= 0.3 voxel, kept within 1k-20k steps per TR."""
+    phantom = dict(dims=(256, 256, 256), fov_um=(256.0, 256.0, 256.0), specs=[cylinders(8.0, 4.0, seed=4)])
+    scales = np.logspace(math.log10(1 / 8), math.log10(100 / 8), 32)
+    D = 1e-9
+    tiles = []
+    for seq, TE in (("gre", 20000), ("se", 40000)):
+        for s in scales:
+            ideal = (0.3 * s * 1e-6) ** 2 / (2 * D) * 1e6                 # us
+            lo, hi = -(-TE // 20000), TE // 1000
+            dt = _divisor_at_most(TE, int(min(max(ideal, lo), hi)), lo)
+            kw = dict(TR=TE + dt, TE=[TE], TIME_STEP=dt, NUMBER_OF_SPINS=100000, T1=[2200.0, 2200.0],
+                      T2=[41.0, 41.0], DIFFUSIVITY=[D, D], SCALE=[float(np.float32(s))], WHAT_TO_SCALE=0)
+            if seq == "gre":
+                cfg = _base(2, RF_FA=[90.0], RF_PH=[0.0], RF_T=[0], **kw)
+            else:
+                cfg = _base(2, RF_FA=[90.0, 180.0], RF_PH=[0.0, 90.0], RF_T=[0, TE // 2], **kw)
+            tiles.append(dict(seq=seq, scale=float(s), dt=dt, cfg=cfg))
+    return dict(name="c4_fov_sweep_256cube_32scales_gre_se_1e5", phantom=phantom, tiles=tiles)
+
+
+WORKLOADS = {"c1": c1, "c2": c2, "c4": c4}
 
 
 def build(engine, wl, formats=(E.SW_FIELD_F32,), n_spins=None):
@@ -78,3 +115,22 @@ def build(engine, wl, formats=(E.SW_FIELD_F32,), n_spins=None):
     p.fieldmap_exist = 1
     case.mask = None
     return ph, case, vf
+
+
+def build_tiles(engine, wl, formats=(E.SW_FIELD_F32,)):
+    """For a workload with tiles: one device phantom and a case per tile.
+    Returns (phantom, [case], vf)."""
+    ph, vf = engine.generate_phantom(wl["phantom"]["dims"], wl["phantom"]["fov_um"], wl["phantom"]["specs"],
+                                     formats=formats)
+    dims = wl["phantom"]["dims"]
+    out = []
+    for t in wl["tiles"]:
+        case = E.prepare_case(dict(t["cfg"]), np.zeros((1, 1, 1), np.uint8), None, ph.fov_m, XYZ0=None)
+        p = case.params
+        for i in range(3):
+            p.phantom_size[i] = dims[i]
+        p.matrix_length = int(np.prod(dims))
+        p.fieldmap_exist = 1
+        case.mask = None
+        out.append(case)
+    return ph, out, vf
