@@ -50,6 +50,16 @@ SW_API int  sw_run_config(sw_engine *eng, const char *config_path);
  * spatial_order 0/1. */
 SW_API int  sw_run_config_ex(sw_engine *eng, const char *config_path, int32_t rng, int32_t precision,
                              int32_t field_format, int32_t spatial_order);
+/* sw_run_config_ex with every option.  With RECORD_TRAJECTORY the spins of a
+ * scale are walked in chunks of trajectory_chunk_spins (0: as many as fit in
+ * 512 MiB of trajectory), through two device buffers in turn; each chunk's
+ * trajectory is downloaded on a second stream while the next chunk walks.  The
+ * results do not depend on the chunk size. */
+typedef struct sw_run_options {
+    int32_t  rng, precision, field_format, spatial_order;
+    uint32_t trajectory_chunk_spins;
+} sw_run_options;
+SW_API int  sw_run_config_opts(sw_engine *eng, const char *config_path, const sw_run_options *opt);
 /* sim::handler::execute (src/sim/handler.cu:8-17).  use_cpu != 0 is rejected with
  * SW_E_INVALID: this engine has no CPU path (the reference's `-p` switch). */
 SW_API int  sw_sim_execute(int use_cpu, uint32_t device_id, const char *const *config_files,

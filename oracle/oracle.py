@@ -273,7 +273,8 @@ def sim(case, spin_begin=0, spin_end=None, n_threads=8, f64=False, timing=False)
         if "M1_init" in case else np.zeros((n, n_te, 3), np.float64 if f64 else np.float32)
     T = np.array(case["T_init"], np.uint8, copy=True) if "T_init" in case else \
         np.zeros((n, n_te), np.uint8)
-    XYZ1 = np.zeros((n, traj, 3), np.float32)
+    XYZ1 = np.array(case["XYZ1_init"], np.float32, copy=True).reshape(n, traj, 3) if "XYZ1_init" in case else \
+        np.zeros((n, traj, 3), np.float32)
     lost = np.zeros(n, np.uint8)
     mask = np.ascontiguousarray(case["mask"], dtype=np.uint8)
     M0 = np.ascontiguousarray(case["M0"], dtype=np.float32)

@@ -46,6 +46,33 @@ struct DevBuf {
 
 struct RunOpts {
     int32_t rng = SW_RNG_PHILOX, precision = SW_PREC_F32, field_format = SW_FIELD_F32, spatial_order = 1;
+    uint32_t traj_chunk_spins = 0;
+};
+
+// Pinned host memory (hipHostMalloc), freed on scope exit.
+struct PinnedBuf {
+    void *p = nullptr;
+    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
+    hipError_t alloc(size_t n) { return n ? hipHostMalloc(&p, n, hipHostMallocDefault) : hipSuccess; }
+};
+
+struct StreamPair {
+    hipStream_t walk = nullptr, copy = nullptr;
+    hipEvent_t walked[2] = {nullptr, nullptr}, copied[2] = {nullptr, nullptr};
+    ~StreamPair() {
+        for (hipEvent_t e : {walked[0], walked[1], copied[0], copied[1]}) if (e) (void)hipEventDestroy(e);
+        if (walk) (void)hipStreamDestroy(walk);
+        if (copy) (void)hipStreamDestroy(copy);
+    }
+    hipError_t create() {
+        hipError_t e = hipStreamCreateWithFlags(&walk, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&copy, hipStreamNonBlocking);
+        for (int i = 0; i < 2 && e == hipSuccess; i++) {
+            e = hipEventCreateWithFlags(&walked[i], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&copied[i], hipEventDisableTiming);
+        }
+        return e;
+    }
 };
 
 int run_phantom(sw_engine *eng, swhost::Config &c, size_t iph, std::vector<float> &M1, std::vector<float> &XYZ1,
@@ -105,47 +132,78 @@ int run_phantom(sw_engine *eng, swhost::Config &c, size_t iph, std::vector<float
     const size_t n_te = c.TE.size(), n_sc = c.scales.size();
     const size_t traj = c.p.enRecordTrajectory ? (size_t)(c.p.n_dummy_scan + 1) * c.p.n_timepoints : 1;
     const size_t m1_slice = 3 * n_te * (size_t)n, t_slice = n_te * (size_t)n, x_slice = 3 * (size_t)n * traj;
-    DevBuf dM0, dX0, dM1, dX1, dT, dRN, dRU;
+    // Recorded trajectories stream: spins are walked in chunks into two device
+    // buffers in turn, and each chunk's trajectory goes back through pinned memory
+    // on a second stream while the next chunk walks.  Final positions alone
+    // (traj == 1) stay on the device for every scale.
+    const bool streamed = c.p.enRecordTrajectory != 0;
+    uint32_t chunk = n;
+    if (streamed) {
+        const uint64_t per_spin = 12ull * traj;
+        const uint64_t cap = ro.traj_chunk_spins ? ro.traj_chunk_spins : std::max<uint64_t>(1, (512ull << 20) / per_spin);
+        chunk = (uint32_t)std::min<uint64_t>(n, cap);
+    }
+    const size_t xc = 3 * (size_t)chunk * traj;                       // floats per chunk buffer
+    DevBuf dM0, dX0, dM1, dX1, dT, dRN, dRU, dXa, dXb;
     size_t fr = 0, tot = 0;
     (void)hipMemGetInfo(&fr, &tot);
-    const size_t need = 24 * (size_t)n + 4 * m1_slice * n_sc + t_slice * n_sc + 4 * x_slice * n_sc;
+    const size_t need = 24 * (size_t)n + 4 * m1_slice * n_sc + t_slice * n_sc +
+                        (streamed ? 2 * 4 * xc : 4 * x_slice * n_sc);
     if (need > fr) return fail(SW_E_NOMEM, "not enough device memory for the spin buffers");
-    if (dM0.alloc(M0.size() * 4) || dX0.alloc(XYZ0.size() * 4) || dM1.alloc(M1.size() * 4) ||
-        dX1.alloc(XYZ1.size() * 4) || dT.alloc(T.size()))
+    if (dM0.alloc(M0.size() * 4) || dX0.alloc(XYZ0.size() * 4) || dM1.alloc(M1.size() * 4) || dT.alloc(T.size()) ||
+        (streamed ? (dXa.alloc(4 * xc) || dXb.alloc(4 * xc)) : dX1.alloc(XYZ1.size() * 4)))
         return fail(SW_E_HIP, "hipMalloc failed");
     // the device outputs start from the host buffers (the reference's d_M1 = M1)
     if (hipMemcpy(dM0.p, M0.data(), M0.size() * 4, hipMemcpyHostToDevice) ||
         hipMemcpy(dX0.p, XYZ0.data(), XYZ0.size() * 4, hipMemcpyHostToDevice) ||
         (M1.size() && hipMemcpy(dM1.p, M1.data(), M1.size() * 4, hipMemcpyHostToDevice)) ||
-        hipMemcpy(dX1.p, XYZ1.data(), XYZ1.size() * 4, hipMemcpyHostToDevice) ||
+        (!streamed && hipMemcpy(dX1.p, XYZ1.data(), XYZ1.size() * 4, hipMemcpyHostToDevice)) ||
         (T.size() && hipMemcpy(dT.p, T.data(), T.size(), hipMemcpyHostToDevice)))
         return fail(SW_E_HIP, "hipMemcpy failed");
+    PinnedBuf pin[2];
+    StreamPair sp;
+    if (streamed && (pin[0].alloc(4 * xc) || pin[1].alloc(4 * xc) || sp.create()))
+        return fail(SW_E_HIP, "trajectory staging buffers");
 
     sw_walk_opts o{};
-    o.n_local = n;
     o.rng = ro.rng;
     o.precision = ro.precision == SW_PREC_F64 ? SW_PREC_F32 : ro.precision;   // M is written as float32
     o.field_format = has_field ? ro.field_format : SW_FIELD_NONE;
     o.spatial_order = ro.spatial_order;
+    uint64_t rn_stride = 0, ru_stride = 0;
     if (ro.rng == SW_RNG_REPLAY) {
         // the reference CPU path's own draws, enough for every attempt but a
         // pathological number of rejections
         const uint64_t steps = (uint64_t)(c.p.n_dummy_scan + 1) * c.p.n_timepoints;
-        o.replay_normal_stride = 9 * steps + 96;
-        o.replay_uniform_stride = 3 * steps + 64;
-        std::vector<float> rn(o.replay_normal_stride * n), ru(o.replay_uniform_stride * n);
-        sw_replay_fill(c.p.seed, 0, n, rn.data(), o.replay_normal_stride, ru.data(), o.replay_uniform_stride, 16);
+        rn_stride = 9 * steps + 96;
+        ru_stride = 3 * steps + 64;
+        std::vector<float> rn(rn_stride * n), ru(ru_stride * n);
+        sw_replay_fill(c.p.seed, 0, n, rn.data(), rn_stride, ru.data(), ru_stride, 16);
         if (dRN.alloc(rn.size() * 4) || dRU.alloc(ru.size() * 4) ||
             hipMemcpy(dRN.p, rn.data(), rn.size() * 4, hipMemcpyHostToDevice) ||
             hipMemcpy(dRU.p, ru.data(), ru.size() * 4, hipMemcpyHostToDevice))
             return fail(SW_E_HIP, "replay upload failed");
-        o.replay_normals = (const float *)dRN.p;
-        o.replay_uniforms = (const float *)dRU.p;
+        o.replay_normal_stride = rn_stride;
+        o.replay_uniform_stride = ru_stride;
     }
     sw_sequence *seq = nullptr;
+    std::unique_ptr<sw_sequence *, void (*)(sw_sequence **)> seq_guard(&seq, [](sw_sequence **q) {
+        if (*q) sw_sequence_destroy(*q);
+    });
     const sw_seq q = c.seq_view();
     auto t0 = std::chrono::steady_clock::now();
     uint64_t lost = 0, steps = 0;
+    // one trajectory chunk in flight per buffer: (scale, first spin, spins)
+    struct Pending { bool live = false; size_t is = 0; uint32_t s0 = 0, cnt = 0; } pend[2];
+    auto land = [&](int b) -> int {                  // chunk in buffer b -> host XYZ1
+        if (!pend[b].live) return SW_OK;
+        if (hipEventSynchronize(sp.copied[b]) != hipSuccess) return fail(SW_E_HIP, "trajectory download");
+        std::memcpy(XYZ1.data() + x_slice * pend[b].is + 3 * (size_t)pend[b].s0 * traj, pin[b].p,
+                    12 * (size_t)pend[b].cnt * traj);
+        pend[b].live = false;
+        return SW_OK;
+    };
+    size_t k = 0;                                                             // chunk counter
     for (size_t is = 0; is < n_sc; is++) {                                     // :273-337
         const float scale = c.scales[is];
         sw_params p = c.p;
@@ -165,22 +223,58 @@ int run_phantom(sw_engine *eng, swhost::Config &c, size_t iph, std::vector<float
             rc = sw_sequence_create(eng, &p, &q, &seq);
             if (rc) return fail(rc, std::string("sequence upload: ") + sw_last_error(eng));
         }
-        sw_walk_stats st{};
-        rc = sw_walk_device(eng, ph, seq, &p, &o, (const float *)dM0.p, (const float *)dX0.p,
-                            (float *)dM1.p + m1_slice * is, (float *)dX1.p + x_slice * is, (uint8_t *)dT.p + t_slice * is,
-                            nullptr, &st);
-        if (rc) { sw_sequence_destroy(seq); return fail(rc, std::string("walk: ") + sw_last_error(eng)); }
-        lost += st.n_lost;
-        steps += st.n_steps;
-        info("scale " + std::to_string(scale) + ": " + std::to_string(st.n_steps) + " spin-steps in " +
-             std::to_string(st.kernel_ms) + " ms, " + std::to_string(st.n_lost) + " spins lost");
+        for (uint32_t s0 = 0; s0 < n || (n == 0 && s0 == 0); s0 += chunk, k++) {
+            const uint32_t cnt = n ? std::min<uint32_t>(chunk, n - s0) : 0;
+            o.spin_offset = s0;
+            o.n_local = cnt;
+            if (ro.rng == SW_RNG_REPLAY) {
+                o.replay_normals = (const float *)dRN.p + rn_stride * s0;
+                o.replay_uniforms = (const float *)dRU.p + ru_stride * s0;
+            }
+            const float *m0 = (const float *)dM0.p + 3 * (size_t)s0, *x0 = (const float *)dX0.p + 3 * (size_t)s0;
+            float *m1 = (float *)dM1.p + m1_slice * is + 3 * n_te * (size_t)s0;
+            uint8_t *tt = (uint8_t *)dT.p + t_slice * is + n_te * (size_t)s0;
+            sw_walk_stats st{};
+            if (!streamed) {
+                rc = sw_walk_device(eng, ph, seq, &p, &o, m0, x0, m1, (float *)dX1.p + x_slice * is + 3 * (size_t)s0,
+                                    tt, nullptr, &st);
+            } else {
+                const int b = (int)(k & 1);
+                rc = land(b);                                   // the chunk that used this buffer before
+                if (rc) return rc;
+                // stale host contents first (abandoned spins keep them), then walk
+                const size_t bytes = 12 * (size_t)cnt * traj;
+                std::memcpy(pin[b].p, XYZ1.data() + x_slice * is + 3 * (size_t)s0 * traj, bytes);
+                float *dx = (float *)(b ? dXb.p : dXa.p);
+                if (hipMemcpyAsync(dx, pin[b].p, bytes, hipMemcpyHostToDevice, sp.walk) != hipSuccess)
+                    return fail(SW_E_HIP, "trajectory upload");
+                rc = sw_walk_device(eng, ph, seq, &p, &o, m0, x0, m1, dx, tt, sp.walk, &st);
+                if (rc == SW_OK) {
+                    if (hipEventRecord(sp.walked[b], sp.walk) != hipSuccess ||
+                        hipStreamWaitEvent(sp.copy, sp.walked[b], 0) != hipSuccess ||
+                        hipMemcpyAsync(pin[b].p, dx, bytes, hipMemcpyDeviceToHost, sp.copy) != hipSuccess ||
+                        hipEventRecord(sp.copied[b], sp.copy) != hipSuccess)
+                        return fail(SW_E_HIP, "trajectory download");
+                    pend[b].live = true; pend[b].is = is; pend[b].s0 = s0; pend[b].cnt = cnt;
+                }
+            }
+            if (rc) return fail(rc, std::string("walk: ") + sw_last_error(eng));
+            lost += st.n_lost;
+            steps += st.n_steps;
+            if (n == 0) break;
+        }
+        info("scale " + std::to_string(scale) + ": " + std::to_string(steps) + " spin-steps so far, " +
+             std::to_string(lost) + " spins lost so far");
     }
-    sw_sequence_destroy(seq);
+    for (int b = 0; b < 2; b++) {
+        rc = land(b);
+        if (rc) return rc;
+    }
     const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     info("Simulation took " + std::to_string(el) + " seconds (" + std::to_string(steps / std::max(el, 1e-9)) +
          " spin-steps/s).");
     if ((M1.size() && hipMemcpy(M1.data(), dM1.p, M1.size() * 4, hipMemcpyDeviceToHost)) ||
-        hipMemcpy(XYZ1.data(), dX1.p, XYZ1.size() * 4, hipMemcpyDeviceToHost) ||
+        (!streamed && hipMemcpy(XYZ1.data(), dX1.p, XYZ1.size() * 4, hipMemcpyDeviceToHost)) ||
         (T.size() && hipMemcpy(T.data(), dT.p, T.size(), hipMemcpyDeviceToHost)))
         return fail(SW_E_HIP, "result download failed");
     (void)lost;
@@ -290,6 +384,17 @@ SW_API int sw_run_config_ex(sw_engine *eng, const char *path, int32_t rng, int32
     ro.precision = precision;
     ro.field_format = field_format;
     ro.spatial_order = spatial_order;
+    return run_config(eng, path, ro);
+}
+
+SW_API int sw_run_config_opts(sw_engine *eng, const char *path, const sw_run_options *opt) {
+    if (!eng || !path || !opt) return SW_E_INVALID;
+    RunOpts ro;
+    ro.rng = opt->rng;
+    ro.precision = opt->precision;
+    ro.field_format = opt->field_format;
+    ro.spatial_order = opt->spatial_order;
+    ro.traj_chunk_spins = opt->trajectory_chunk_spins;
     return run_config(eng, path, ro);
 }
 

@@ -21,7 +21,7 @@ _lib = None
 
 EXPORTS = ("sw_config_load", "sw_config_destroy", "sw_config_free_string", "sw_config_params", "sw_config_scales",
            "sw_config_file", "sw_config_n_files", "sw_config_output", "sw_config_seq_name", "sw_run_config",
-           "sw_run_config_ex", "sw_sim_execute", "sw_h5_read", "sw_h5_write", "sw_generate_config",
+           "sw_run_config_ex", "sw_run_config_opts", "sw_sim_execute", "sw_h5_read", "sw_h5_write", "sw_generate_config",
            "sw_generate_pgse", "sw_host_last_error")
 
 
@@ -50,6 +50,7 @@ def load():
     L.sw_config_seq_name.restype = C.c_char_p
     L.sw_run_config.argtypes = [_P, C.c_char_p]
     L.sw_run_config_ex.argtypes = [_P, C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    L.sw_run_config_opts.argtypes = [_P, C.c_char_p, C.POINTER(SwRunOptions)]
     L.sw_sim_execute.argtypes = [C.c_int, C.c_uint32, C.POINTER(C.c_char_p), C.c_uint32]
     L.sw_h5_read.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, _P, C.c_uint64, C.POINTER(C.c_uint64),
                              C.POINTER(C.c_uint32)]
@@ -65,6 +66,11 @@ def load():
 
 class ConfigError(ValueError):
     pass
+
+
+class SwRunOptions(C.Structure):
+    _fields_ = [("rng", C.c_int32), ("precision", C.c_int32), ("field_format", C.c_int32),
+                ("spatial_order", C.c_int32), ("trajectory_chunk_spins", C.c_uint32)]
 
 
 def _arr(ptr, n, dt):
@@ -148,9 +154,10 @@ def write_phantom(path, mask, field_T, fov_m):
 
 
 def run_config(engine, path, rng=E.SW_RNG_PHILOX, precision=E.SW_PREC_F32, field_format=E.SW_FIELD_F32,
-               spatial_order=1):
+               spatial_order=1, trajectory_chunk_spins=0):
     L = load()
-    rc = L.sw_run_config_ex(engine.h, str(path).encode(), rng, precision, field_format, spatial_order)
+    o = SwRunOptions(rng, precision, field_format, spatial_order, trajectory_chunk_spins)
+    rc = L.sw_run_config_opts(engine.h, str(path).encode(), C.byref(o))
     if rc != E.SW_OK:
         raise E.EngineError(f"sw_run_config: {L.sw_host_last_error().decode()} ({rc})")
 

@@ -28,7 +28,8 @@ def _check(got_M, got_T, got_X, o):
     assert (np.abs(got_M - o["M1"]) <= 1e-5 * mag + 1e-7).all()
 
 
-def test_run_config_replay_matches_oracle(engine, tmp_path):
+@pytest.mark.parametrize("record,chunk", [(0, 0), (1, 0), (1, 37)])
+def test_run_config_replay_matches_oracle(engine, tmp_path, record, chunk):
     fov_um = 24.0
     m1, f1 = cases.cylinder_phantom(n=12, fov_um=fov_um, radius_um=2.0, n_cyl=3, seed=21)
     m2, f2 = cases.three_substrate_phantom(n=12, fov_um=fov_um, seed=22)
@@ -37,11 +38,15 @@ def test_run_config_replay_matches_oracle(engine, tmp_path):
     H.write_phantom(tmp_path / "ph2.h5", m2, f2, fov)
     # stuck spins (MAX_ITERATIONS=2 with impermeable pockets) exercise the stale outputs
     cfg = cases.se(3, TE=3000, dt=50, NUMBER_OF_SPINS=200, MAX_ITERATIONS=2, DIFFUSIVITY=[2e-7, 2e-7, 2e-7],
-                   P_XY=[[1, 0, 0.5], [0, 1, 0.5], [0.5, 0.5, 1]], WHAT_TO_SCALE=0, SCALE=[0.5, 1.0])
+                   P_XY=[[1, 0, 0.5], [0, 1, 0.5], [0.5, 0.5, 1]], WHAT_TO_SCALE=0, SCALE=[0.5, 1.0],
+                   RECORD_TRAJECTORY=record)
     cfg["TE"] = [1500, 3000]
+    traj = 3050 // 50 if record else 1
     ini = tmp_path / "se.ini"
     cases.write_config(ini, cfg, ["ph1.h5", "ph2.h5"], output_dir="out", seq_name="se")
-    H.run_config(engine, ini, rng=E.SW_RNG_REPLAY)
+    # with trajectories the spins of a scale stream through the device in chunks
+    # (a chunk of 37 spins leaves a ragged last chunk)
+    H.run_config(engine, ini, rng=E.SW_RNG_REPLAY, trajectory_chunk_spins=chunk)
 
     prev = None
     for k, (m, f) in enumerate(((m1, f1), (m2, f2))):
@@ -49,7 +54,7 @@ def test_run_config_replay_matches_oracle(engine, tmp_path):
         M = H.h5_read(out, "M")
         T = H.h5_read(out, "T", np.uint8)
         X = H.h5_read(out, "XYZ")
-        assert M.shape == (2, 200, 2, 3) and T.shape == (2, 200, 2, 1) and X.shape == (2, 200, 1, 3)
+        assert M.shape == (2, 200, 2, 3) and T.shape == (2, 200, 2, 1) and X.shape == (2, 200, traj, 3)
         np.testing.assert_array_equal(H.h5_read(out, "scales").ravel(), np.float32([0.5, 1.0]))
         np.testing.assert_array_equal(H.h5_read(out, "TE").ravel(), np.float32([1500e-6, 3000e-6]))
         case = cases.make_case(cfg, m, f, fov_um)
@@ -57,12 +62,12 @@ def test_run_config_replay_matches_oracle(engine, tmp_path):
         for si in range(2):
             d = cases.to_oracle(case, si)
             if prev is not None:
-                d["M1_init"], d["T_init"] = prev[0][si], prev[1][si]
+                d["M1_init"], d["T_init"], d["XYZ1_init"] = prev[0][si], prev[1][si], prev[2][si]
             o = orc.sim(d)
             lost += o["n_lost"]
             _check(M[si], T[si][..., 0], X[si], o)
         assert lost > 0
-        prev = (M, T[..., 0])
+        prev = (M, T[..., 0], X)
 
 
 def test_cli_runs_a_config(tmp_path):
