@@ -203,6 +203,10 @@ int run_phantom(sw_engine *eng, swhost::Config &c, size_t iph, std::vector<float
         pend[b].live = false;
         return SW_OK;
     };
+    // Without recorded trajectories every scale of the phantom is one tile of a
+    // single batched launch (the same results as one walk per scale).
+    const bool batched = !streamed && n_sc > 1 && n > 0;
+    std::vector<sw_batch_item> tiles;
     size_t k = 0;                                                             // chunk counter
     for (size_t is = 0; is < n_sc; is++) {                                     // :273-337
         const float scale = c.scales[is];
@@ -235,6 +239,15 @@ int run_phantom(sw_engine *eng, swhost::Config &c, size_t iph, std::vector<float
             float *m1 = (float *)dM1.p + m1_slice * is + 3 * n_te * (size_t)s0;
             uint8_t *tt = (uint8_t *)dT.p + t_slice * is + n_te * (size_t)s0;
             sw_walk_stats st{};
+            if (batched) {
+                sw_batch_item it;
+                std::memset(&it, 0, sizeof it);
+                it.phantom = ph; it.sequence = seq; it.params = p; it.opts = o;
+                it.d_M0 = m0; it.d_XYZ0 = x0; it.d_M1 = m1; it.d_T = tt;
+                it.d_XYZ1 = (float *)dX1.p + x_slice * is + 3 * (size_t)s0;
+                tiles.push_back(it);
+                break;
+            }
             if (!streamed) {
                 rc = sw_walk_device(eng, ph, seq, &p, &o, m0, x0, m1, (float *)dX1.p + x_slice * is + 3 * (size_t)s0,
                                     tt, nullptr, &st);
@@ -263,8 +276,22 @@ int run_phantom(sw_engine *eng, swhost::Config &c, size_t iph, std::vector<float
             steps += st.n_steps;
             if (n == 0) break;
         }
-        info("scale " + std::to_string(scale) + ": " + std::to_string(steps) + " spin-steps so far, " +
-             std::to_string(lost) + " spins lost so far");
+        if (!batched)
+            info("scale " + std::to_string(scale) + ": " + std::to_string(steps) + " spin-steps so far, " +
+                 std::to_string(lost) + " spins lost so far");
+    }
+    if (batched) {
+        sw_batch *bt = nullptr;
+        rc = sw_batch_create(eng, tiles.data(), (uint32_t)tiles.size(), &bt);
+        if (rc) return fail(rc, std::string("batch: ") + sw_last_error(eng));
+        sw_walk_stats tot{};
+        rc = sw_batch_run(bt, nullptr, nullptr, &tot);
+        sw_batch_destroy(bt);
+        if (rc) return fail(rc, std::string("walk: ") + sw_last_error(eng));
+        lost += tot.n_lost;
+        steps += tot.n_steps;
+        info(std::to_string(n_sc) + " scales in one launch: " + std::to_string(steps) + " spin-steps in " +
+             std::to_string(tot.kernel_ms) + " ms, " + std::to_string(lost) + " spins lost");
     }
     for (int b = 0; b < 2; b++) {
         rc = land(b);
