@@ -145,6 +145,11 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
     ref_sig = [_signals(it["echo_sums"].cpu().numpy()) for it in items64]
     err_b = max((_max_rel(a, r) or 0.0) for a, r in zip(b_sig, ref_sig))
     err_s = max((_max_rel(a, r) or 0.0) for a, r in zip(seq_sig, ref_sig))
+    per_tile = [dict(tile=i, scale=float(wl["tiles"][i]["scale"]),
+                     signals={f"echo{te}_label{lab}": dict(value=a[(te, lab)], f64=r[(te, lab)],
+                                                           rel_err=abs(a[(te, lab)] - r[(te, lab)]) / r[(te, lab)])
+                              for (te, lab) in r if (te, lab) in a and r[(te, lab)] > 0})
+                for i, (a, r) in enumerate(zip(b_sig, ref_sig))]
     log(f"[bench] max relative signal error against FP64: batched {err_b:.2e}, plain FP32 {err_s:.2e}")
     bpe = 5 if fmt == E.SW_FIELD_F32 else 3
     achieved = total_steps * bpe / (st["kernel_ms"] * 1e-3) / 1e9
@@ -154,7 +159,8 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
         "scaling": "weak", "vs_baseline": None,
         "dtype": {0: "f32", 1: "f32-kahan", 2: "f64"}[prec] + ("" if fmt == 1 else f"+{args.field}-field"),
         "data": "synthetic",
-        "config": {"workload": wl["name"], "phantom": "256^3 voxels, 256 um, random cylinders r8 4%",
+        "config": {"workload": wl["name"], "phantom": "x".join(str(d) for d in wl["phantom"]["dims"]) +
+                   f" voxels, {wl['phantom']['fov_um'][0]:g} um, random cylinders",
                    "tiles": len(tiles), "spins_per_tile": tiles[0]["n_local"], "spin_steps": total_steps,
                    "precision": args.precision, "field": args.field, "mode": "batched, one HIP graph",
                    "rng": "philox4x32-10"},
@@ -163,7 +169,7 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
                      "kernel_ms": st["kernel_ms"]},
         "sequential_f32": {"value": total_steps / seq_s, "ms_per_step": seq_s * 1e3,
                            "signal_error_vs_f64": err_s},
-        "signal_error_vs_f64": {"max_rel": err_b},
+        "signal_error_vs_f64": {"max_rel": err_b, "per_tile": per_tile if len(tiles) <= 8 else None},
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -207,7 +213,7 @@ def main():
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
     eng = E.Engine(0)
-    if args.workload == "c4":
+    if "tiles" in W.WORKLOADS[args.workload]():
         return main_tiles(args, torch, E, W, eng, stream, dev)
     wl = W.WORKLOADS[args.workload]()
     fmt = FIELDS[args.field]

@@ -93,7 +93,22 @@ def c4():
     return dict(name="c4_fov_sweep_256cube_32scales_gre_se_1e5", phantom=phantom, tiles=tiles)
 
 
-WORKLOADS = {"c1": c1, "c2": c2, "c4": c4}
+def c3():
+    """Precision form: 1000^3 voxels over 1000 um (1 GB mask) of random cylinders
+    (r 5 um, 4 %), 1e7 spins, bSSFP (TR 5 ms, TE 2.5 ms, FA 16, dt 50 us) with 100
+    dummy scans and 4-step linear phase cycling (LINEAR_PHASE_CYCLING 90 scaled by
+    0, 1, 2, 3; WHAT_TO_SCALE = 2): 4 tiles of 1e7 spins x 10 100 steps."""
+    phantom = dict(dims=(1000, 1000, 1000), fov_um=(1000.0, 1000.0, 1000.0), specs=[cylinders(5.0, 4.0, seed=3)])
+    tiles = []
+    for k in range(4):
+        cfg = _base(2, TR=5000, TE=[2500], RF_FA=[16.0], RF_PH=[0.0], RF_T=[0], TIME_STEP=50, DUMMY_SCAN=100,
+                    LINEAR_PHASE_CYCLING=90.0, WHAT_TO_SCALE=2, SCALE=[float(k)], NUMBER_OF_SPINS=10000000,
+                    T1=[2200.0, 2200.0], T2=[41.0, 41.0])
+        tiles.append(dict(seq="bssfp", scale=float(k), dt=50, cfg=cfg))
+    return dict(name="c3_bssfp_1000cube_1e7_4cycles", phantom=phantom, tiles=tiles)
+
+
+WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4}
 
 
 def build(engine, wl, formats=(E.SW_FIELD_F32,), n_spins=None):
