@@ -86,13 +86,13 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
     wl = W.WORKLOADS[args.workload]()
     fmt, prec = FIELDS[args.field], PRECISIONS[args.precision]
     log(f"[bench] workload {wl['name']}: generating phantom on the device")
-    ph, cases_, vf = W.build_tiles(eng, wl, formats=(E.SW_FIELD_F32, fmt))
+    phs, cases_, pidx, vf = W.build_tiles(eng, wl, formats=(E.SW_FIELD_F32, fmt))
     tiles = []
-    for case in cases_:
+    for case, k in zip(cases_, pidx):
         p, xs, gs = case.scale_params(case.scales[0])
         n = case.n_spins
         z = lambda *sh, dt=torch.float32: torch.zeros(sh, dtype=dt, device=dev)
-        tiles.append(dict(phantom=ph, sequence=eng.sequence(p, case.tables), params=p, xyz_scale=xs,
+        tiles.append(dict(phantom=phs[k], sequence=eng.sequence(p, case.tables), params=p, xyz_scale=xs,
                           gradient_scale=gs, spin_offset=0, n_local=n, spatial_order=args.order,
                           M0=torch.from_numpy(case.M0).to(dev), XYZ0=torch.from_numpy(case.XYZ0).to(dev),
                           M1=z(n, case.n_te, 3), T=z(n, case.n_te, dt=torch.uint8), XYZ1=z(n, case.traj_len, 3),
@@ -151,6 +151,18 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
                               for (te, lab) in r if (te, lab) in a and r[(te, lab)] > 0})
                 for i, (a, r) in enumerate(zip(b_sig, ref_sig))]
     log(f"[bench] max relative signal error against FP64: batched {err_b:.2e}, plain FP32 {err_s:.2e}")
+    rows = []
+    for i, (a, r) in enumerate(zip(b_sig, ref_sig)):
+        for key in r:
+            if key in a and r[key] > 0:
+                rows.append(dict(tile=i, seq=wl["tiles"][i]["seq"], scale=float(wl["tiles"][i]["scale"]),
+                                 echo=key[0], label=key[1], signal=a[key], f64=r[key],
+                                 abs_err=abs(a[key] - r[key]), rel_err=abs(a[key] - r[key]) / r[key]))
+    worst = sorted(rows, key=lambda x: -x["rel_err"])[:5]
+    for w in worst:
+        log(f"[bench]   tile {w['tile']} {w['seq']} scale {w['scale']:.3g} label {w['label']}: signal "
+            f"{w['signal']:.6g} vs FP64 {w['f64']:.6g}, abs {w['abs_err']:.2e}, rel {w['rel_err']:.2e}")
+    max_abs = max((w["abs_err"] for w in rows), default=None)
     bpe = 5 if fmt == E.SW_FIELD_F32 else 3
     achieved = total_steps * bpe / (st["kernel_ms"] * 1e-3) / 1e9
     out = {
@@ -159,8 +171,10 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
         "scaling": "weak", "vs_baseline": None,
         "dtype": {0: "f32", 1: "f32-kahan", 2: "f64"}[prec] + ("" if fmt == 1 else f"+{args.field}-field"),
         "data": "synthetic",
-        "config": {"workload": wl["name"], "phantom": "x".join(str(d) for d in wl["phantom"]["dims"]) +
-                   f" voxels, {wl['phantom']['fov_um'][0]:g} um, random cylinders",
+        "config": {"workload": wl["name"],
+                   "phantom": ("x".join(str(d) for d in wl["phantom"]["dims"]) +
+                               f" voxels, {wl['phantom']['fov_um'][0]:g} um, random cylinders")
+                   if "phantom" in wl else f"{len(phs)} phantoms of cylinders, spheres and packed axons",
                    "tiles": len(tiles), "spins_per_tile": tiles[0]["n_local"], "spin_steps": total_steps,
                    "precision": args.precision, "field": args.field, "mode": "batched, one HIP graph",
                    "rng": "philox4x32-10"},
@@ -169,12 +183,13 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
                      "kernel_ms": st["kernel_ms"]},
         "sequential_f32": {"value": total_steps / seq_s, "ms_per_step": seq_s * 1e3,
                            "signal_error_vs_f64": err_s},
-        "signal_error_vs_f64": {"max_rel": err_b, "per_tile": per_tile if len(tiles) <= 8 else None},
+        "signal_error_vs_f64": {"max_rel": err_b, "max_abs": max_abs, "worst": worst,
+                                "per_tile": per_tile if len(tiles) <= 8 else None},
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"[bench] CPU baseline: oracle on {threads} threads, {args.cpu_seconds} s budget, first tile")
-        out["cpu_baseline"] = cpu_baseline(cases_[0], ph, args.cpu_seconds, threads)
+        out["cpu_baseline"] = cpu_baseline(cases_[0], phs[pidx[0]], args.cpu_seconds, threads)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -171,6 +171,9 @@ SW_API int  sw_phantom_download(const sw_phantom *ph, uint8_t *mask, float *fiel
  * (nullable) receives the analytic volume fraction placed per spec, percent. */
 #define SW_SHAPE_CYLINDER   0
 #define SW_SHAPE_SPHERE     1
+#define SW_SHAPE_AXONS      2   /* the reference cylinder generator's geometry: parallel
+                                   to z, B0 tilted by orientation_deg in the x-z plane,
+                                   optional random radii shrunk to fit (packed axons) */
 typedef struct sw_shape_spec {
     int32_t  kind;                /* SW_SHAPE_*                                       */
     uint32_t label;               /* mask value inside the shape (1..255)             */
@@ -180,7 +183,8 @@ typedef struct sw_shape_spec {
     float    orientation_deg;     /* cylinders: axis tilt from B0 otherwise           */
     float    dchi, Y;             /* field scale 2pi/4pi * (1-Y) * dchi              */
     uint32_t seed;
-    float    trunc_radii;         /* field truncation, in radii (0: cylinders none, spheres 10) */
+    float    trunc_radii;         /* field truncation, in radii (0: cylinders none, spheres 10, axons 20) */
+    float    radius_min_um;       /* axons: > 0, or radius_um < 0, draws radii in [radius_min, |radius|] */
 } sw_shape_spec;
 SW_API int  sw_phantom_generate(sw_engine *eng, const uint64_t dims[3], const float fov_um[3],
                                 const sw_shape_spec *specs, uint32_t n_specs, uint32_t formats,

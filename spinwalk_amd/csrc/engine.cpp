@@ -444,13 +444,18 @@ SW_API int sw_phantom_generate(sw_engine *e, const uint64_t dims[3], const float
     if (!e || !dims || !fov_um || (n_specs && !specs) || !out) return fail(e, SW_E_INVALID, "null argument");
     *out = nullptr;
     SW_HIP(e, hipSetDevice(e->device));
-    std::vector<sw::GenCylinder> cyl;
+    std::vector<sw::GenCylinder> cyl, axn;
     std::vector<sw::GenSphere> sph;
     for (uint32_t i = 0; i < n_specs; i++) {
         sw::ShapeSpec ss;
         ss.kind = specs[i].kind;
         if (specs[i].label == 0 || specs[i].label > 255) return fail(e, SW_E_INVALID, "shape label must be 1..255");
-        if (!(specs[i].radius_um > 0)) return fail(e, SW_E_INVALID, "shape radius must be positive");
+        const bool neg_ok = ss.kind == SW_SHAPE_AXONS;         // negative radius: random radii
+        if (!(specs[i].radius_um > 0 || (neg_ok && specs[i].radius_um < 0)))
+            return fail(e, SW_E_INVALID, "shape radius must be positive");
+        if (specs[i].radius_min_um < 0 || (specs[i].radius_min_um > 0 && !neg_ok))
+            return fail(e, SW_E_INVALID, "radius_min_um applies to axons and must not be negative");
+        ss.radius_min_um = specs[i].radius_min_um;
         ss.label = (uint8_t)specs[i].label;
         ss.radius_um = specs[i].radius_um;
         ss.volume_fraction = specs[i].volume_fraction;
@@ -463,6 +468,7 @@ SW_API int sw_phantom_generate(sw_engine *e, const uint64_t dims[3], const float
         double vf = 0;
         if (ss.kind == SW_SHAPE_CYLINDER) vf = sw::sample_cylinders(ss, fov_um, cyl);
         else if (ss.kind == SW_SHAPE_SPHERE) vf = sw::sample_spheres(ss, fov_um, cyl, sph);
+        else if (ss.kind == SW_SHAPE_AXONS) vf = sw::sample_axons(ss, fov_um, axn);
         else return fail(e, SW_E_INVALID, "unknown shape kind");
         if (achieved_vf) achieved_vf[i] = vf;
     }
@@ -485,6 +491,24 @@ SW_API int sw_phantom_generate(sw_engine *e, const uint64_t dims[3], const float
         std::vector<uint32_t> fillp(start.begin(), start.end() - 1);
         for (size_t k = 0; k < sph.size(); k++) sorted[fillp[cid[k]]++] = sph[k];
     }
+    // 2-D cell list of axons over the x-y plane, cells as large as every truncation radius
+    float acell = 1.f;
+    for (auto &a : axn) acell = std::max(acell, std::sqrt(a.trunc2));
+    const uint32_t anc[2] = {std::max(1u, (uint32_t)std::ceil(fov_um[0] / acell)),
+                             std::max(1u, (uint32_t)std::ceil(fov_um[1] / acell))};
+    std::vector<uint32_t> astart((size_t)anc[0] * anc[1] + 1, 0), acid(axn.size());
+    for (size_t k = 0; k < axn.size(); k++) {
+        const uint32_t cx = std::min(anc[0] - 1, (uint32_t)std::max(0.f, axn[k].p[0] / acell));
+        const uint32_t cy = std::min(anc[1] - 1, (uint32_t)std::max(0.f, axn[k].p[1] / acell));
+        acid[k] = cx * anc[1] + cy;
+        astart[acid[k] + 1]++;
+    }
+    for (size_t c = 0; c + 1 < astart.size(); c++) astart[c + 1] += astart[c];
+    std::vector<sw::GenCylinder> asorted(axn.size());
+    {
+        std::vector<uint32_t> fillp(astart.begin(), astart.end() - 1);
+        for (size_t k = 0; k < axn.size(); k++) asorted[fillp[acid[k]]++] = axn[k];
+    }
     std::unique_ptr<sw_phantom> ph(new sw_phantom);
     ph->eng = e;
     ph->device = e->device;
@@ -498,12 +522,15 @@ SW_API int sw_phantom_generate(sw_engine *e, const uint64_t dims[3], const float
     hipStream_t s = e->stream;
     sw::GenCylinder *dcyl = nullptr;
     sw::GenSphere *dsph = nullptr;
-    uint32_t *dstart = nullptr;
+    uint32_t *dstart = nullptr, *dastart = nullptr;
+    sw::GenCylinder *daxn = nullptr;
     SW_HIP(e, hipMalloc((void **)&ph->mask, ph->n));
     SW_HIP(e, hipMalloc((void **)&ph->field, ph->n * sizeof(float)));
     SW_HIP(e, dev_upload(&dcyl, cyl.data(), cyl.size(), s));
     SW_HIP(e, dev_upload(&dsph, sorted.data(), sorted.size(), s));
     SW_HIP(e, dev_upload(&dstart, start.data(), start.size(), s));
+    SW_HIP(e, dev_upload(&daxn, asorted.data(), asorted.size(), s));
+    SW_HIP(e, dev_upload(&dastart, astart.data(), astart.size(), s));
     sw::GenArgs g;
     std::memset(&g, 0, sizeof g);
     g.n = ph->n;
@@ -514,12 +541,16 @@ SW_API int sw_phantom_generate(sw_engine *e, const uint64_t dims[3], const float
     g.cell_start = dstart;
     for (int i = 0; i < 3; i++) g.nc[i] = nc[i];
     g.cell = cell;
+    g.axn = daxn; g.n_axn = (uint32_t)asorted.size();
+    g.axn_start = dastart;
+    g.anc[0] = anc[0]; g.anc[1] = anc[1];
+    g.acell = acell;
     g.background = 0;
     g.mask = ph->mask;
     g.field = ph->field;
     SW_HIP(e, sw::launch_phantom_gen(g, s));
     SW_HIP(e, hipStreamSynchronize(s));
-    dev_free(dcyl); dev_free(dsph); dev_free(dstart);
+    dev_free(dcyl); dev_free(dsph); dev_free(dstart); dev_free(daxn); dev_free(dastart);
     rc = phantom_finish(e, ph.get(), formats);
     if (rc) { sw_phantom_destroy(ph.release()); return rc; }
     *out = ph.release();

@@ -6,8 +6,8 @@
 // (src/phantom/phantom_sphere.cpp:79-198), each adding the closed-form field of
 // a magnetised cylinder or sphere to every voxel of a bounding box.  Here every
 // voxel is one lane that sums the closed forms of all cylinders (any axis) and
-// of the spheres in the neighbouring cells of a cell list, and writes the label
-// of the shape that contains it.  Fields are in tesla per 1 T of B0 (B0 along z):
+// of the spheres and of the axons (cylinders parallel to z) in the neighbouring
+// cells of their cell lists, and writes the label of the shape that contains it.  Fields are in tesla per 1 T of B0 (B0 along z):
 //   cylinder, outside: 2 pi (1-Y) dChi (a^2/r^2) cos(2 phi) sin^2(theta)
 //   cylinder, inside:  2 pi (1-Y) dChi (cos^2(theta) - 1/3)
 //   sphere, outside:   4 pi (1-Y) dChi (a^3/r^3) (cos^2(phi) - 1/3), 0 inside
@@ -68,6 +68,30 @@ __global__ void gen_kernel(GenArgs g) {
                             const float inv = 1.f / r2;
                             field += s.k_a3 * inv * sqrtf(inv) * (dz * dz * inv - (1.f / 3.f));
                         }
+                    }
+                }
+            }
+        }
+    }
+    if (g.n_axn) {
+        const int cx = min((int)(x / g.acell), (int)g.anc[0] - 1);
+        const int cyi = min((int)(y / g.acell), (int)g.anc[1] - 1);
+        for (int ox = -1; ox <= 1; ox++) {
+            const int ax = cx + ox;
+            if (ax < 0 || ax >= (int)g.anc[0]) continue;
+            for (int oy = -1; oy <= 1; oy++) {
+                const int ay = cyi + oy;
+                if (ay < 0 || ay >= (int)g.anc[1]) continue;
+                const uint32_t cell = (uint32_t)ax * g.anc[1] + (uint32_t)ay;
+                for (uint32_t k = g.axn_start[cell]; k < g.axn_start[cell + 1]; k++) {
+                    const GenCylinder cy = g.axn[k];
+                    const float qx = x - cy.p[0], qy = y - cy.p[1];
+                    const float r2 = qx * qx + qy * qy;
+                    if (r2 <= cy.a2) {
+                        label = cy.label;
+                        field += cy.k_in;
+                    } else if (r2 < cy.trunc2) {
+                        field += cy.k_out * (2.f * qx * qx / r2 - 1.f) / r2;   // B0 projection along x
                     }
                 }
             }

@@ -35,6 +35,11 @@ struct GenArgs {
     const uint32_t *cell_start;  // [ncells+1], spheres sorted by cell
     uint32_t nc[3];
     float cell;                  // cell edge (um) >= sphere truncation radius
+    // axons: cylinders parallel to z, sorted by their cell in the x-y plane
+    const GenCylinder *axn; uint32_t n_axn;
+    const uint32_t *axn_start;   // [anc[0]*anc[1]+1]
+    uint32_t anc[2];
+    float acell;                 // cell edge (um) >= axon truncation radius
     uint8_t background;
     uint8_t *mask;
     float *field;
@@ -54,10 +59,12 @@ struct ShapeSpec {
     float dchi, Y;
     uint32_t seed;
     float trunc_radii;          // field truncation in radii (0: default)
+    float radius_min_um;        // > 0 (or radius_um < 0): random radii, shrunk to fit
 };
 
 double sample_cylinders(const ShapeSpec &s, const float fov_um[3], std::vector<GenCylinder> &out);
 double sample_spheres(const ShapeSpec &s, const float fov_um[3], const std::vector<GenCylinder> &cyl,
                       std::vector<GenSphere> &out);
+double sample_axons(const ShapeSpec &s, const float fov_um[3], std::vector<GenCylinder> &out);
 
 }  // namespace sw

@@ -149,4 +149,74 @@ double sample_spheres(const ShapeSpec &s, const float L[3], const std::vector<Ge
     return 100.0 * vol / box;
 }
 
+// Axons: the reference cylinder generator's geometry (src/phantom/phantom_cylinder.cpp:
+// 85-130, 183-275): cylinders parallel to z, B0 tilted from z by orientation_deg
+// in the x-z plane, centres uniform in the x-y square, rejected when they fall
+// inside an accepted axon.  With random radii (radius_um < 0, or radius_min_um > 0)
+// a radius is drawn in [radius_min, |radius|] and shrunk to the gap when it
+// overlaps a neighbour (check_cylinder_overlap, :24-60); a radius shrunk below
+// radius_min is rejected.  The field is truncated at trunc_radii radii (default
+// 20, the reference's bounding box of 20 radii).  A 2-D cell list of edge
+// 2 |radius| keeps the overlap test local.
+double sample_axons(const ShapeSpec &s, const float L[3], std::vector<GenCylinder> &out) {
+    std::mt19937 gen(s.seed);
+    std::uniform_real_distribution<double> uni(0.0, 1.0);
+    const double rmax = std::fabs(s.radius_um);
+    const bool rnd = s.radius_um < 0 || s.radius_min_um > 0;
+    const double rmin = s.radius_min_um > 0 ? std::min((double)s.radius_min_um, rmax) : 0.0;
+    const double area = (double)L[0] * L[1];
+    const double target = s.volume_fraction / 100.0 * area;
+    const double k = 2 * kPi * (1.0 - s.Y) * s.dchi;
+    const double th = s.orientation_deg * kPi / 180.0;
+    const double ct = std::cos(th), st2 = 1.0 - ct * ct;
+    const double trr = s.trunc_radii > 0 ? s.trunc_radii : 20.0;
+    const double cell = 2.0 * rmax;
+    const int ncx = std::max(1, (int)std::ceil(L[0] / cell)), ncy = std::max(1, (int)std::ceil(L[1] / cell));
+    std::vector<std::vector<uint32_t>> grid((size_t)ncx * ncy);
+    struct Disk { double x, y, a; };
+    std::vector<Disk> disks;
+    double acc = 0;
+    long misses = 0;
+    // stop at the target, or after 2e6 draws in a row that place nothing (jammed)
+    while (acc < target && misses < 2000000) {
+        const double x = uni(gen) * L[0], y = uni(gen) * L[1];
+        double a = rnd ? rmin + uni(gen) * (rmax - rmin) : rmax;
+        const int cx = std::min((int)(x / cell), ncx - 1), cy = std::min((int)(y / cell), ncy - 1);
+        bool ok = a > 0;
+        for (int ox = -1; ox <= 1 && ok; ox++)
+            for (int oy = -1; oy <= 1 && ok; oy++) {
+                const int gx = cx + ox, gy = cy + oy;
+                if (gx < 0 || gy < 0 || gx >= ncx || gy >= ncy) continue;
+                for (uint32_t j : grid[(size_t)gx * ncy + gy]) {
+                    const double d = std::hypot(x - disks[j].x, y - disks[j].y);
+                    if (d <= disks[j].a) { ok = false; break; }
+                    if (d < disks[j].a + a) {
+                        if (!rnd) { ok = false; break; }
+                        a = d - disks[j].a;
+                        if (a < rmin || a <= 0) { ok = false; break; }
+                    }
+                }
+            }
+        if (!ok) { misses++; continue; }
+        misses = 0;
+        grid[(size_t)cx * ncy + cy].push_back((uint32_t)disks.size());
+        disks.push_back({x, y, a});
+        acc += kPi * a * a;
+    }
+    for (const Disk &dk : disks) {
+        GenCylinder c{};
+        c.p[0] = (float)dk.x; c.p[1] = (float)dk.y; c.p[2] = 0.f;
+        c.u[0] = 0.f; c.u[1] = 0.f; c.u[2] = 1.f;
+        c.bp[0] = 1.f; c.bp[1] = 0.f; c.bp[2] = 0.f;              // B0 projected on the x-y plane
+        c.a2 = (float)(dk.a * dk.a);
+        c.k_in = (float)(k * (ct * ct - 1.0 / 3.0));
+        c.k_out = (float)(k * dk.a * dk.a * st2);
+        // parallel to B0 there is no outside field: the lookup then only needs the radius
+        c.trunc2 = st2 > 0 ? (float)(trr * trr * dk.a * dk.a) : c.a2;
+        c.label = s.label;
+        out.push_back(c);
+    }
+    return 100.0 * acc / area;
+}
+
 }  // namespace sw

@@ -96,6 +96,7 @@ struct PhaseF32 {          // reference: float accumulated_phase (kernels.cu:126
     __device__ void add(float x) { v += x; }
     __device__ void add_grad(double g) { v = (float)((double)v + g); }   // kernels.cu:185
     __device__ float value() const { return v; }
+    __device__ float angle() const { return v; }     // what the rotation at an event uses
     __device__ void reset() { v = 0.f; }
 };
 struct PhaseKahan {        // FP32 with Kahan compensation
@@ -109,6 +110,9 @@ struct PhaseKahan {        // FP32 with Kahan compensation
     }
     __device__ void add_grad(double g) { add((float)g); }
     __device__ float value() const { return v - c; }
+    // The rotation angle reduced modulo 360 before the compensation is applied
+    // (fmodf is exact), so a phase of thousands of degrees keeps its low bits.
+    __device__ float angle() const { return fmodf(v, 360.f) - c; }
     __device__ void reset() { v = 0.f; c = 0.f; }
 };
 struct PhaseF64 {
@@ -117,14 +121,10 @@ struct PhaseF64 {
     __device__ void add(double x) { v += x; }
     __device__ void add_grad(double g) { v += g; }
     __device__ double value() const { return v; }
+    __device__ double angle() const { return v; }
     __device__ void reset() { v = 0.; }
 };
 
-// ---------------------------------------------------------------------------
-// Field gathers.  The reference multiplies the fieldmap by the T->deg/step
-// factor in float once on the host (monte_carlo.cu:243-244); multiplying the
-// gathered value by the same float factor gives the same float.
-// ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
 // Voxel records.  The reference multiplies the fieldmap by the T->deg/step factor
 // in float once on the host (monte_carlo.cu:243-244); multiplying the stored
@@ -451,7 +451,7 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
                 }
                 if (tp == next_rf) {                               // :190-199
                     const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
-                    dephase_relax(m, ph.value(), T1, T2, te);
+                    dephase_relax(m, ph.angle(), T1, T2, te);
                     const RfPulse p = a.rf[rf_i];
                     if (sizeof(R) == 8) rot_apply<R>(p.kind_d, (R)p.sd, (R)p.cd, (R)p.spd, (R)p.cpd, m);
                     else                rot_apply<R>(p.kind, (R)p.s, (R)p.c, (R)p.sp, (R)p.cp, m);
@@ -462,7 +462,7 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
                 }
                 if (tp == next_te) {                               // :202-215
                     const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
-                    dephase_relax(m, ph.value(), T1, T2, te);
+                    dephase_relax(m, ph.angle(), T1, T2, te);
                     const uint64_t o = (uint64_t)li * a.n_te + te_i;
                     R *M1 = (R *)a.M1;
                     M1[3 * o + 0] = m[0]; M1[3 * o + 1] = m[1]; M1[3 * o + 2] = m[2];
@@ -483,7 +483,7 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
             ++tp;
         }
         const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);   // :226-231
-        dephase_relax(m, ph.value(), T1, T2, te);
+        dephase_relax(m, ph.angle(), T1, T2, te);
     }
     if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
     n_acc += (uint64_t)(a.n_dummy + 1) * a.n_timepoints;

@@ -82,11 +82,11 @@ class SwShapeSpec(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("label", C.c_uint32), ("radius_um", C.c_float), ("volume_fraction", C.c_float),
         ("random_orientation", C.c_int32), ("orientation_deg", C.c_float), ("dchi", C.c_float), ("Y", C.c_float),
-        ("seed", C.c_uint32), ("trunc_radii", C.c_float),
+        ("seed", C.c_uint32), ("trunc_radii", C.c_float), ("radius_min_um", C.c_float),
     ]
 
 
-SW_SHAPE_CYLINDER, SW_SHAPE_SPHERE = 0, 1
+SW_SHAPE_CYLINDER, SW_SHAPE_SPHERE, SW_SHAPE_AXONS = 0, 1, 2
 
 
 class SwWalkStats(C.Structure):

@@ -108,7 +108,76 @@ def c3():
     return dict(name="c3_bssfp_1000cube_1e7_4cycles", phantom=phantom, tiles=tiles)
 
 
-WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4}
+def axons(radius_max_um, radius_min_um, vf, seed, label=1, orientation_deg=0.0):
+    """Packed parallel axons: random radii in [min, max] shrunk to fit, along z."""
+    return dict(kind=E.SW_SHAPE_AXONS, label=label, radius_um=radius_max_um, volume_fraction=vf,
+                random_orientation=0, orientation_deg=orientation_deg, dchi=DCHI, Y=Y_OXY, seed=seed,
+                trunc_radii=10.0, radius_min_um=radius_min_um)
+
+
+GAMMA = 267515315.0
+
+
+def pgse_tables(b_first, axis, start_ms, delta_ms, DELTA_ms, dt):
+    """The PGSE RF pair and gradient waveform written by the `dwi` subcommand
+    (src/dwi/pgse.cpp:78-135), in the reference's INI keys."""
+    d, D = delta_ms * 1e-3, DELTA_ms * 1e-3
+    G = math.sqrt(b_first * 1e6 / (GAMMA * GAMMA * d * d * (D - d / 3.0))) * 1000.0
+    u = np.asarray(axis, float) / np.linalg.norm(axis)
+    start_us = start_ms * 1000
+    n = delta_ms * 1000 // dt
+    amp = [0.0] + [G] * n + [0.0] + [0.0] + [G] * n + [0.0]
+    gap = (DELTA_ms - delta_ms) * 1000
+    t = [start_us - dt] + [start_us + i * dt for i in range(n)] + [start_us + n * dt + dt]
+    t += [start_us + gap + n * dt - dt] + [start_us + gap + i * dt for i in range(n, 2 * n)]
+    t += [start_us + gap + 2 * n * dt + dt]
+    amp = np.asarray(amp)
+    return dict(RF_FA=[90.0, 180.0], RF_PH=[0.0, 90.0],
+                RF_T=[0, start_us + delta_ms * 1000 + (DELTA_ms - delta_ms) * 1000 // 2],
+                GRADIENT_X=(amp * u[0]).tolist(), GRADIENT_Y=(amp * u[1]).tolist(),
+                GRADIENT_Z=(amp * u[2]).tolist(), GRADIENT_T=t)
+
+
+def c5():
+    """Heterogeneous batch: 64 phantoms of 128^3-384^3 voxels (1 um voxels) of
+    random cylinders, spheres or packed parallel axons (r 0.5-3 um, 30-70 %), each
+    walked with one of GRE (TE 20 ms), SE (TE 40 ms) or PGSE along x, y or z
+    (delta 10 ms, Delta 30 ms, b = 1000, 0, 2000, 3000 s/mm^2 as gradient scales of
+    the first), with 1e4-1e6 spins (log-uniform), dt 50 us.  Every tile of every
+    phantom goes into one launch."""
+    rng = np.random.default_rng(2024)
+    phantoms, tiles = [], []
+    for i in range(64):
+        n = int(rng.choice([128, 192, 256, 320, 384]))
+        kind = i % 3
+        if kind == 0:
+            spec = cylinders(float(rng.uniform(2.0, 6.0)), float(rng.uniform(2.0, 6.0)), seed=100 + i)
+        elif kind == 1:
+            spec = spheres(float(rng.uniform(2.0, 5.0)), float(rng.uniform(2.0, 6.0)), seed=100 + i, label=1)
+        else:
+            spec = axons(3.0, 0.5, float(rng.uniform(30.0, 70.0)), seed=100 + i)
+        phantoms.append(dict(dims=(n, n, n), fov_um=(float(n),) * 3, specs=[spec]))
+        spins = int(round(10 ** rng.uniform(4.0, 6.0)))
+        seq = ["gre", "se", "pgse_x", "pgse_y", "pgse_z"][int(rng.integers(0, 5))]
+        common = dict(NUMBER_OF_SPINS=spins, TIME_STEP=50, T1=[2200.0, 2200.0], T2=[41.0, 41.0],
+                      DIFFUSIVITY=[1e-9, 1e-9], P_XY=[[1.0, 0.5], [0.5, 1.0]], SEED=1000 + i)
+        if seq == "gre":
+            cfgs = [_base(2, TR=20050, TE=[20000], RF_FA=[90.0], RF_PH=[0.0], RF_T=[0], **common)]
+        elif seq == "se":
+            cfgs = [_base(2, TR=40050, TE=[40000], RF_FA=[90.0, 180.0], RF_PH=[0.0, 90.0], RF_T=[0, 20000],
+                          **common)]
+        else:
+            axis = {"pgse_x": (1, 0, 0), "pgse_y": (0, 1, 0), "pgse_z": (0, 0, 1)}[seq]
+            tab = pgse_tables(1000.0, axis, 5, 10, 30, 50)
+            TE = 2 * (5000 + 10000 + 10000)
+            cfgs = [_base(2, TR=TE + 50, TE=[TE], WHAT_TO_SCALE=1, SCALE=[float(np.float32(math.sqrt(b / 1000.0)))],
+                          **tab, **common) for b in (1000.0, 0.0, 2000.0, 3000.0)]
+        for cfg in cfgs:
+            tiles.append(dict(seq=seq, scale=float(cfg["SCALE"][0]), dt=50, cfg=cfg, phantom=i))
+    return dict(name="c5_hetero_64_phantoms", phantoms=phantoms, tiles=tiles)
+
+
+WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5}
 
 
 def build(engine, wl, formats=(E.SW_FIELD_F32,), n_spins=None):
@@ -133,14 +202,19 @@ def build(engine, wl, formats=(E.SW_FIELD_F32,), n_spins=None):
 
 
 def build_tiles(engine, wl, formats=(E.SW_FIELD_F32,)):
-    """For a workload with tiles: one device phantom and a case per tile.
-    Returns (phantom, [case], vf)."""
-    ph, vf = engine.generate_phantom(wl["phantom"]["dims"], wl["phantom"]["fov_um"], wl["phantom"]["specs"],
-                                     formats=formats)
-    dims = wl["phantom"]["dims"]
-    out = []
+    """For a workload with tiles: the device phantoms and a case per tile.
+    Returns ([phantom], [case], [phantom index per tile], [vf])."""
+    specs = wl["phantoms"] if "phantoms" in wl else [wl["phantom"]]
+    phs, vfs = [], []
+    for sp in specs:
+        ph, vf = engine.generate_phantom(sp["dims"], sp["fov_um"], sp["specs"], formats=formats)
+        phs.append(ph)
+        vfs.append(vf)
+    out, idx = [], []
     for t in wl["tiles"]:
-        case = E.prepare_case(dict(t["cfg"]), np.zeros((1, 1, 1), np.uint8), None, ph.fov_m, XYZ0=None)
+        k = t.get("phantom", 0)
+        dims = specs[k]["dims"]
+        case = E.prepare_case(dict(t["cfg"]), np.zeros((1, 1, 1), np.uint8), None, phs[k].fov_m, XYZ0=None)
         p = case.params
         for i in range(3):
             p.phantom_size[i] = dims[i]
@@ -148,4 +222,5 @@ def build_tiles(engine, wl, formats=(E.SW_FIELD_F32,)):
         p.fieldmap_exist = 1
         case.mask = None
         out.append(case)
-    return ph, out, vf
+        idx.append(k)
+    return phs, out, idx, vfs
