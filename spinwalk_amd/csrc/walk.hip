@@ -311,14 +311,31 @@ __device__ __forceinline__ Vox<IDX32> locate(const WalkArgs &a, const double x[3
     return v;
 }
 
+// The boundary correction of one axis (kernels.cu:133-136) for v = xo + rw:
+// v < 0: v + adj; v >= fov: v - adj, which rounds as v + (-adj); else v.
 template <bool CROSS>
-__device__ __forceinline__ double move_axis(double fov, float r, double diff, double xo) {
-    const double rw = (double)r * diff;                          // kernels.cu:131
-    const double v = xo + rw;
-    const double adj = CROSS ? fov : 2 * fabs(rw);               // :133-136
-    // v < 0: v + adj; v >= fov: v - adj, which rounds as v + (-adj); else v.
+__device__ __forceinline__ double fix_axis(double fov, double rw, double v) {
+    const double adj = CROSS ? fov : 2 * fabs(rw);
     const double sadj = v < 0 ? adj : -adj;
     return (v < 0 || v >= fov) ? v + sadj : v;
+}
+
+// A step of all three axes (kernels.cu:130-137).  Spins rarely reach a wall, so
+// the correction runs only when some lane of the wave needs it.
+template <bool CROSS>
+__device__ __forceinline__ void move(const WalkArgs &a, float px, float py, float pz, double diff, const double xo[3],
+                                     double xn[3]) {
+    const double rw0 = (double)px * diff, rw1 = (double)py * diff, rw2 = (double)pz * diff;   // :131
+    xn[0] = xo[0] + rw0;
+    xn[1] = xo[1] + rw1;
+    xn[2] = xo[2] + rw2;
+    const bool out = xn[0] < 0 || xn[0] >= a.fov[0] || xn[1] < 0 || xn[1] >= a.fov[1] || xn[2] < 0 ||
+                     xn[2] >= a.fov[2];
+    if (__builtin_expect(__ballot(out) != 0, 0)) {
+        xn[0] = fix_axis<CROSS>(a.fov[0], rw0, xn[0]);
+        xn[1] = fix_axis<CROSS>(a.fov[1], rw1, xn[1]);
+        xn[2] = fix_axis<CROSS>(a.fov[2], rw2, xn[2]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -387,9 +404,7 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
             const bool moved = diff != 0.;
             if (moved) {                                           // :130-137
                 if (!rng.avail) { n_acc += (uint64_t)d * a.n_timepoints + tp; return ST_EXHAUSTED; }
-                xn[0] = move_axis<CROSS>(a.fov[0], rng.px, diff, xo[0]);
-                xn[1] = move_axis<CROSS>(a.fov[1], rng.py, diff, xo[1]);
-                xn[2] = move_axis<CROSS>(a.fov[2], rng.pz, diff, xo[2]);
+                move<CROSS>(a, rng.px, rng.py, rng.pz, diff, xo, xn);
             }
             vx = locate<Rec, IDX32>(a, xn);                        // :140
             if (!vx.ok) {                                          // :141-147
