@@ -320,8 +320,16 @@ __device__ __forceinline__ double fix_axis(double fov, double rw, double v) {
     return (v < 0 || v >= fov) ? v + sadj : v;
 }
 
+// v < 0 or v >= fov, or v is -0.0 (fov > 0 finite): the bit patterns of the
+// non-negative doubles, read as unsigned integers, are ordered as the values,
+// and every pattern with the sign bit set lies above them.  One 64-bit compare
+// instead of two FP64 compares; fix_axis makes the exact decision.
+__device__ __forceinline__ bool maybe_outside(double v, double fov) {
+    return (uint64_t)__double_as_longlong(v) >= (uint64_t)__double_as_longlong(fov);
+}
+
 // A step of all three axes (kernels.cu:130-137).  Spins rarely reach a wall, so
-// the correction runs only when some lane of the wave needs it.
+// the correction runs only when some lane of the wave may need it.
 template <bool CROSS>
 __device__ __forceinline__ void move(const WalkArgs &a, float px, float py, float pz, double diff, const double xo[3],
                                      double xn[3]) {
@@ -329,8 +337,8 @@ __device__ __forceinline__ void move(const WalkArgs &a, float px, float py, floa
     xn[0] = xo[0] + rw0;
     xn[1] = xo[1] + rw1;
     xn[2] = xo[2] + rw2;
-    const bool out = xn[0] < 0 || xn[0] >= a.fov[0] || xn[1] < 0 || xn[1] >= a.fov[1] || xn[2] < 0 ||
-                     xn[2] >= a.fov[2];
+    const bool out = maybe_outside(xn[0], a.fov[0]) | maybe_outside(xn[1], a.fov[1]) |
+                     maybe_outside(xn[2], a.fov[2]);
     if (__builtin_expect(__ballot(out) != 0, 0)) {
         xn[0] = fix_axis<CROSS>(a.fov[0], rw0, xn[0]);
         xn[1] = fix_axis<CROSS>(a.fov[1], rw1, xn[1]);
@@ -380,8 +388,12 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
     rng.peek();
     Phase ph;
     uint64_t slot = 0;
+    // Every early end (lost, stuck or out of draws) leaves through `done`, so the
+    // step loop has one exit edge besides its natural end.
+    int st = ST_OK;
+    uint32_t d = 0, tp = 0;
 
-    for (uint32_t d = 0; d <= a.n_dummy; ++d) {                    // :107
+    for (d = 0; d <= a.n_dummy; ++d) {                              // :107
         const bool last = (d == a.n_dummy);
         {                                                          // :110-120 excitation
             const Excitation e = a.exc[d];
@@ -391,7 +403,8 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
                 rot_apply<R>(e.kind, (R)a.exc_s, (R)a.exc_c, (R)e.sp, (R)e.cp, m);
         }
         RecIdx<IDX32> rec_old = ~(RecIdx<IDX32>)0;                 // :123-126 (ind_old: no voxel)
-        uint32_t tp = 0, old_tp = 0;
+        uint32_t old_tp = 0;
+        tp = 0;
         uint32_t rf_i = 1, te_i = 0, dp_i = 0, gr_i = 0;
         uint32_t next_rf = (a.n_rf > 1) ? (uint32_t)a.rf[1].t : kNever;
         uint32_t next_te = (last && a.n_te > 0) ? (uint32_t)a.TE_t[0] : kNever;
@@ -403,15 +416,11 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
         while (tp < a.n_timepoints) {                              // :128
             const bool moved = diff != 0.;
             if (moved) {                                           // :130-137
-                if (!rng.avail) { n_acc += (uint64_t)d * a.n_timepoints + tp; return ST_EXHAUSTED; }
+                if (!rng.avail) { st = ST_EXHAUSTED; goto done; }
                 move<CROSS>(a, rng.px, rng.py, rng.pz, diff, xo, xn);
             }
             vx = locate<Rec, IDX32>(a, xn);                        // :140
-            if (!vx.ok) {                                          // :141-147
-                if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
-                n_acc += (uint64_t)d * a.n_timepoints + tp;
-                return ST_LOST;
-            }
+            if (!vx.ok) { st = ST_LOST; goto done; }               // :141-147
             const bool newvox = vx.rec != rec_old;                 // :150
             typename Rec::T r{};
             if (newvox)                                            // one gather: label and field
@@ -425,14 +434,10 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
                 const R fnew = Rec::template field<R>(a, r);       // :166, read speculatively
                 if (ts != ts_old) {                                // :153-163
                     float u;
-                    if (!rng.uniform(u)) { n_acc += (uint64_t)d * a.n_timepoints + tp; return ST_EXHAUSTED; }
+                    if (!rng.uniform(u)) { st = ST_EXHAUSTED; goto done; }
                     if (u >= a.pXY[ts_old * a.n_sub + ts]) {
                         ++n_rej;
-                        if (itr++ > a.max_iterations) {
-                            if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
-                            n_acc += (uint64_t)d * a.n_timepoints + tp;
-                            return ST_LOST;
-                        }
+                        if (itr++ > a.max_iterations) { st = ST_LOST; goto done; }
                         continue;
                     }
                     ts_old = ts;
@@ -500,9 +505,15 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
         const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);   // :226-231
         dephase_relax(m, ph.angle(), T1, T2, te);
     }
+    d = a.n_dummy + 1;                                             // every TR done
+    tp = 0;
+done:
+    // Without trajectories XYZ1 holds the last accepted position: the reference
+    // stores every accepted position in place (kernels.cu:222-223), and a spin
+    // that is lost keeps the one before.  Then the count of accepted steps.
     if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
-    n_acc += (uint64_t)(a.n_dummy + 1) * a.n_timepoints;
-    return ST_OK;
+    n_acc += (uint64_t)d * a.n_timepoints + tp;
+    return st;
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
