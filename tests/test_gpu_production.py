@@ -150,6 +150,34 @@ def test_free_diffusion_increments(engine):
     assert np.abs(c - np.eye(3)).max() < 5 / math.sqrt(n)
 
 
+def test_walk_consumes_the_materialized_draws(engine):
+    """Every attempted step k of spin s moves by (double)draw[s, k, 0:3] * diff, with
+    draw = sw_philox_draws_device(seed, s, k): replaying the move arithmetic of
+    kernels.cu:131 in float64 on the recorded draws gives the recorded trajectory
+    bit for bit.  One substrate, no walls within reach: every attempt is accepted."""
+    import torch
+    mask = np.zeros((8, 8, 8), np.uint8)
+    cfg = cases.gre(1, TE=5000, dt=50, DIFFUSIVITY=[2e-9], RECORD_TRAJECTORY=1, T1=[1e9], T2=[1e9])
+    rng = np.random.default_rng(3)
+    n = 1000
+    xyz0 = rng.uniform(0.45, 0.55, size=(n, 3)).astype(np.float32) * np.float32(1000e-6)
+    case = cases.make_case(cfg, mask, None, 1000.0, n_spins=n, xyz0=xyz0)
+    g = run(engine, case)
+    n_t = case.traj_len
+    assert g["stats"]["n_lost"] == 0 and n_t > 50
+    d = torch.zeros((n, n_t, 4), dtype=torch.float32, device="cuda")
+    engine.philox_draws(case.params.seed, 0, n, n_t, d)
+    torch.cuda.synchronize()
+    draws = d.cpu().numpy()
+    diff = float(case.tables["diffusivity"][0])
+    x = xyz0.astype(np.float64)
+    want = np.empty((n, n_t, 3), np.float32)
+    for k in range(n_t):
+        x = x + draws[:, k, :3].astype(np.float64) * diff
+        want[:, k] = x.astype(np.float32)
+    np.testing.assert_array_equal(g["XYZ1"], want)
+
+
 def test_pgse_discrete_stejskal_tanner(engine):
     D, b, delta, DELTA, dt = 2e-9, 500.0, 5.0, 10.0, 50
     mask = np.zeros((8, 8, 8), np.uint8)
