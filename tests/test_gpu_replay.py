@@ -185,12 +185,36 @@ def test_replay_odd_dims_wild_wrap_out_of_axis_subscripts(engine):
     _assert_parity(g, o)
 
 
-def test_replay_wide_slab_64bit_index_path(engine):
-    # nx*ny >= 2^24 selects the 64-bit index arithmetic
-    m, f = cases.blocky_phantom((4100, 4100, 1), n_labels=2, seed=4, block=5)
+def test_replay_long_slab_64bit_index_path(engine):
+    # nx+ny+nz > 16384: the per-axis tables would not fit in 64 KiB of LDS, so the
+    # walk takes the 64-bit index arithmetic
+    m, f = cases.blocky_phantom((16400, 3, 3), n_labels=2, seed=4, block=5)
     cfg = cases.gre(2, TE=500, dt=50, P_XY=[[1, 0.5], [0.5, 1]], DIFFUSIVITY=[1e-7, 1e-7])
-    case = cases.make_case(cfg, m, f, (4100.0, 4100.0, 1.0), n_spins=256)
+    case = cases.make_case(cfg, m, f, (16400.0, 3.0, 3.0), n_spins=256)
     rp = _replay_streams(case)
     o = orc.sim(cases.to_oracle(case))
     g = _run_engine(engine, case, replay=rp)
+    _assert_parity(g, o)
+
+
+def test_replay_records_beyond_4gb(engine):
+    """1024^3 voxels with an FP32 field: 8.6 GB of voxel records, beyond 32-bit byte
+    offsets, so the walk takes the 64-bit index arithmetic at full size.  The
+    spins walk near the far x end, where every record lies beyond 4 GB."""
+    n = 1024
+    m = np.zeros((n, n, n), np.uint8)
+    f = np.zeros((n, n, n), np.float32)
+    sub = (slice(960, 1024), slice(480, 544), slice(480, 544))
+    bm, bf = cases.blocky_phantom((64, 64, 64), n_labels=2, seed=9, block=4)
+    m[sub] = bm
+    f[sub] = bf
+    cfg = cases.gre(2, TE=2500, dt=50, P_XY=[[1, 0.5], [0.5, 1]], DIFFUSIVITY=[1e-9, 1e-9])
+    rng = np.random.default_rng(9)
+    lo, hi = np.float32([975, 495, 495]), np.float32([1010, 530, 530])
+    xyz0 = (rng.uniform(0, 1, size=(256, 3)).astype(np.float32) * (hi - lo) + lo) * np.float32(1e-6)
+    case = cases.make_case(cfg, m, f, float(n), n_spins=256, xyz0=xyz0)
+    rp = _replay_streams(case)
+    o = orc.sim(cases.to_oracle(case))
+    g = _run_engine(engine, case, replay=rp)
+    assert g["stats"]["n_lost"] == 0
     _assert_parity(g, o)
