@@ -194,6 +194,48 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
     return 0
 
 
+# Working sets of the probe: the L2 of one XCD, under the 256 MB Infinity Cache,
+# and the FP32 voxel records of the 500^3 and 1000^3 workloads (bricks of 16
+# eight-byte records: 1.0e9 and 8.0e9 bytes).
+PROBE_SIZES = ((4 << 20, "4 MiB"), (192 << 20, "192 MiB"), (1_000_000_000, "1.0 GB (500^3 FP32 records)"),
+               (8_000_000_000, "8.0 GB (1000^3 FP32 records)"))
+
+
+def main_probe(args, torch, eng, stream, dev):
+    """Random-line read rates and the streaming copy rate on this GPU: the
+    ceilings that the walk's gather traffic is compared with."""
+    n_lanes, n_reads = 1 << 20, 128
+    sink = torch.empty(n_lanes, dtype=torch.int64, device=dev)
+    rows = []
+    for nbytes, name in PROBE_SIZES:
+        buf = torch.empty(nbytes // 8, dtype=torch.int64, device=dev)
+        torch.cuda.synchronize()
+        for _ in range(2):
+            eng.probe_gather(buf, n_lanes, n_reads, sink, stream=stream)
+        ms = sorted(eng.probe_gather(buf, n_lanes, n_reads, sink, seed=k + 2, stream=stream) for k in range(5))[2]
+        lines = n_lanes * n_reads
+        rows.append(dict(working_set=name, bytes=nbytes, ms=ms, lines_per_s=lines / (ms * 1e-3),
+                         line_GBps=lines * 128 / (ms * 1e-3) / 1e9))
+        log(f"[probe] random 128-byte lines over {name}: {rows[-1]['line_GBps']:.0f} GB/s of lines "
+            f"({rows[-1]['lines_per_s']:.3g} lines/s)")
+        del buf
+    a = torch.empty(1 << 28, dtype=torch.float32, device=dev)          # 1 GiB
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(10):
+        b.copy_(a)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    copy_GBps = 10 * 2 * a.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    log(f"[probe] 1 GiB device copy: {copy_GBps:.0f} GB/s (read + write)")
+    print(json.dumps({"probe": "memory ceilings", "random_line_reads": rows, "copy_GBps": copy_GBps,
+                      "hbm_peak_GBps": HBM_PEAK_GBS, "n_lanes": n_lanes, "reads_per_lane": n_reads}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +251,9 @@ def main():
                     help="after the timed region, time one host-buffer call (sw_walk_host: uploads, walk, downloads)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed region, run one device copy of this size (profiler calibration)")
+    ap.add_argument("--probe", action="store_true",
+                    help="instead of the walk: the memory ceilings of the gather (random 128-byte line reads at "
+                         "several working-set sizes) and of a streaming copy")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -228,6 +273,8 @@ def main():
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
     eng = E.Engine(0)
+    if args.probe:
+        return main_probe(args, torch, eng, stream, dev)
     if "tiles" in W.WORKLOADS[args.workload]():
         return main_tiles(args, torch, E, W, eng, stream, dev)
     wl = W.WORKLOADS[args.workload]()

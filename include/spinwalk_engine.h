@@ -262,6 +262,13 @@ SW_API int  sw_echo_reduce_device(sw_engine *eng, const void *d_M1, int m1_dtype
  * the generator's statistics; the walk draws exactly these values. */
 SW_API int  sw_philox_draws_device(sw_engine *eng, uint64_t seed, uint32_t spin0, uint32_t n_spins,
                                    uint32_t n_att, float *d_out, void *stream);
+/* The memory ceiling of the voxel gather: n_lanes lanes each read n_reads (a
+ * multiple of 8) 8-byte words at uniformly random 128-byte lines of
+ * d_buf[0, bytes) (bytes >= 128); d_sink[n_lanes] (uint64, device) receives a
+ * checksum that keeps the reads live.  Synchronous; *ms = kernel time on the
+ * stream.  Lines read = n_lanes * n_reads. */
+SW_API int  sw_probe_gather_device(sw_engine *eng, const void *d_buf, uint64_t bytes, uint32_t n_lanes,
+                                   uint32_t n_reads, uint64_t seed, uint64_t *d_sink, void *stream, float *ms);
 /* Waits for `stream` (NULL: the engine's stream). */
 SW_API int  sw_stream_sync(sw_engine *eng, void *stream);
 

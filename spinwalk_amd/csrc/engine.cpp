@@ -1090,6 +1090,21 @@ SW_API int sw_philox_draws_device(sw_engine *e, uint64_t seed, uint32_t spin0, u
     return SW_OK;
 }
 
+SW_API int sw_probe_gather_device(sw_engine *e, const void *d_buf, uint64_t bytes, uint32_t n_lanes,
+                                  uint32_t n_reads, uint64_t seed, uint64_t *d_sink, void *stream, float *ms) {
+    if (!e || !d_buf || !d_sink || !ms) return fail(e, SW_E_INVALID, "null argument");
+    if (bytes < 128 || n_lanes == 0 || n_reads == 0 || n_reads % 8)
+        return fail(e, SW_E_INVALID, "probe needs bytes >= 128, lanes > 0 and reads a positive multiple of 8");
+    hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    SW_HIP(e, hipSetDevice(e->device));
+    SW_HIP(e, hipEventRecord(e->ev0, s));
+    SW_HIP(e, sw::launch_gather_probe(d_buf, bytes, n_lanes, n_reads, seed, d_sink, s));
+    SW_HIP(e, hipEventRecord(e->ev1, s));
+    SW_HIP(e, hipEventSynchronize(e->ev1));
+    SW_HIP(e, hipEventElapsedTime(ms, e->ev0, e->ev1));
+    return SW_OK;
+}
+
 SW_API int sw_stream_sync(sw_engine *e, void *stream) {
     if (!e) return SW_E_INVALID;
     SW_HIP(e, hipStreamSynchronize(stream ? (hipStream_t)stream : e->stream));

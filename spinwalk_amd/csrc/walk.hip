@@ -324,8 +324,8 @@ __device__ __forceinline__ double fix_axis(double fov, double rw, double v) {
 // non-negative doubles, read as unsigned integers, are ordered as the values,
 // and every pattern with the sign bit set lies above them.  One 64-bit compare
 // instead of two FP64 compares; fix_axis makes the exact decision.
-__device__ __forceinline__ bool maybe_outside(double v, double fov) {
-    return (uint64_t)__double_as_longlong(v) >= (uint64_t)__double_as_longlong(fov);
+__device__ __forceinline__ uint32_t maybe_outside(double v, double fov) {
+    return (uint64_t)__double_as_longlong(v) >= (uint64_t)__double_as_longlong(fov) ? 1u : 0u;
 }
 
 // A step of all three axes (kernels.cu:130-137).  Spins rarely reach a wall, so
@@ -337,8 +337,8 @@ __device__ __forceinline__ void move(const WalkArgs &a, float px, float py, floa
     xn[0] = xo[0] + rw0;
     xn[1] = xo[1] + rw1;
     xn[2] = xo[2] + rw2;
-    const bool out = maybe_outside(xn[0], a.fov[0]) | maybe_outside(xn[1], a.fov[1]) |
-                     maybe_outside(xn[2], a.fov[2]);
+    const bool out = (maybe_outside(xn[0], a.fov[0]) | maybe_outside(xn[1], a.fov[1]) |
+                      maybe_outside(xn[2], a.fov[2])) != 0;
     if (__builtin_expect(__ballot(out) != 0, 0)) {
         xn[0] = fix_axis<CROSS>(a.fov[0], rw0, xn[0]);
         xn[1] = fix_axis<CROSS>(a.fov[1], rw1, xn[1]);

@@ -27,6 +27,12 @@ hipError_t launch_walk_batch(const WalkArgs *items, const BatchChunk *chunks, ui
 hipError_t launch_philox_draws(uint64_t seed, uint32_t spin0, uint32_t n_spins, uint32_t n_att,
                                float *out, hipStream_t s);
 
+// Random-line read probe (probe.hip): n_lanes lanes, n_reads (a multiple of 8)
+// 8-byte reads each at uniformly random 128-byte lines of buf[0, bytes);
+// sink[n_lanes] keeps the reads live.
+hipError_t launch_gather_probe(const void *buf, uint64_t bytes, uint32_t n_lanes, uint32_t n_reads, uint64_t seed,
+                               uint64_t *sink, hipStream_t s);
+
 // Phantom preparation: max label and max |field| (into stats[0], stats[1] as
 // uint32 bit patterns; stats must be zeroed by the caller), then half copies.
 hipError_t launch_phantom_stats(const uint8_t *mask, const float *field, uint64_t n,

@@ -106,7 +106,7 @@ EXPORTS = (
     "sw_replay_fill", "sw_phantom_create", "sw_phantom_create_device", "sw_phantom_destroy",
     "sw_phantom_info", "sw_phantom_download", "sw_phantom_generate", "sw_sequence_create", "sw_sequence_destroy", "sw_walk_device", "sw_walk_host",
     "sw_batch_create", "sw_batch_run", "sw_batch_destroy",
-    "sw_echo_reduce_device", "sw_philox_draws_device", "sw_stream_sync",
+    "sw_echo_reduce_device", "sw_philox_draws_device", "sw_probe_gather_device", "sw_stream_sync",
 )
 
 _lib = None
@@ -164,6 +164,8 @@ def load():
                                _P, _P, C.POINTER(SwWalkStats)]
     L.sw_echo_reduce_device.argtypes = [_P, _P, C.c_int, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]
     L.sw_philox_draws_device.argtypes = [_P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]
+    L.sw_probe_gather_device.argtypes = [_P, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, _P, _P,
+                                         C.POINTER(C.c_float)]
     L.sw_stream_sync.argtypes = [_P, _P]
     L.sw_batch_create.argtypes = [_P, C.POINTER(SwBatchItem), C.c_uint32, C.POINTER(_P)]
     L.sw_batch_run.argtypes = [_P, _P, C.POINTER(SwWalkStats), C.POINTER(SwWalkStats)]
@@ -517,6 +519,18 @@ class Engine:
                                                  n_sub, _P(d_out.data_ptr()), sp), "sw_echo_reduce_device")
         if wait:
             self.sync()
+
+    def probe_gather(self, d_buf, n_lanes, n_reads, d_sink, seed=1, stream=None):
+        """Random 128-byte line reads over the tensor d_buf; returns the kernel time
+        in ms.  d_sink: uint64/int64 device tensor of n_lanes entries."""
+        if d_sink.numel() < n_lanes or d_sink.element_size() != 8:
+            raise EngineError("d_sink needs n_lanes 8-byte entries")
+        ms = C.c_float()
+        sp, _ = _stream_arg(stream)
+        self._check(self.L.sw_probe_gather_device(self.h, _P(d_buf.data_ptr()), d_buf.numel() * d_buf.element_size(),
+                                                  int(n_lanes), int(n_reads), int(seed), _P(d_sink.data_ptr()), sp,
+                                                  C.byref(ms)), "sw_probe_gather_device")
+        return ms.value
 
     def philox_draws(self, seed, spin0, n_spins, n_att, d_out, stream=None):
         sp, wait = _stream_arg(stream)
