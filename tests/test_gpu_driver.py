@@ -83,3 +83,43 @@ def test_cli_runs_a_config(tmp_path):
     # -p (the CPU path) is refused
     r = subprocess.run([exe, "sim", "-c", str(tmp_path / "g.ini"), "-p"], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
+
+
+def test_generated_phantoms_and_config_run_end_to_end(engine, tmp_path):
+    """The reference's own simulation test (tests/test_sim.cu:21-43) on this engine:
+    two phantoms of parallel cylinders with random radii up to 20 um (the
+    reference's negative radius), 5 % volume fraction, FoV 600 um on 300^3 voxels,
+    dChi 0.11e-6, Y 0.85 and 0.78, seed 45, along B0; a GRE config from the
+    `config` generator (TE 20 ms, dt 25 us) with NUMBER_OF_SPINS = 1000; run
+    through sim::handler::execute.  Beyond running: the output layout, and
+    magnitudes bounded by T2 decay with the extravascular signal of the phantom
+    with more deoxygenated blood (Y 0.78) the smaller."""
+    import ctypes as C
+    names = []
+    for k, Y in enumerate((0.85, 0.78)):
+        spec = dict(kind=E.SW_SHAPE_AXONS, label=1, radius_um=-20.0, volume_fraction=5.0, random_orientation=0,
+                    orientation_deg=0.0, dchi=0.11e-6, Y=Y, seed=45, trunc_radii=0.0, radius_min_um=0.0)
+        ph, _ = engine.generate_phantom((300, 300, 300), (600.0,) * 3, [spec])
+        m, f = ph.download(field=True)
+        ph.close()
+        assert abs(100.0 * np.count_nonzero(m) / m.size - 5.0) < 2.0     # tests/test_phantom.cpp:15-22
+        names.append(f"phantom{k + 1}.h5")
+        H.write_phantom(tmp_path / names[-1], m, f, np.full(3, np.float32(600e-6)))
+    H.generate_config("GRE", 20000, 25, names, tmp_path / "gre.ini")
+    H.write_ini(tmp_path / "gre_sim.ini", {"GENERAL": {"SEQ_NAME": "gre", "PARENT_CONFIG": "gre.ini"},
+                                           "SIMULATION_PARAMETERS": {"NUMBER_OF_SPINS": 1000}})
+    files = (C.c_char_p * 1)(str(tmp_path / "gre_sim.ini").encode())
+    L = H.load()
+    assert L.sw_sim_execute(0, 0, files, 1) == E.SW_OK, L.sw_host_last_error().decode()
+    c = H.load_config(tmp_path / "gre_sim.ini")
+    signal = []
+    for out in c["outputs"]:
+        M = H.h5_read(out, "M")
+        T = H.h5_read(out, "T", np.uint8)
+        assert M.shape == (1, 1000, 1, 3) and T.shape == (1, 1000, 1, 1)
+        np.testing.assert_array_equal(H.h5_read(out, "TE").ravel(), np.float32([20000e-6]))
+        mxy = np.linalg.norm(M[0, :, 0, :2].astype(np.float64), axis=-1)
+        assert (mxy <= np.exp(-20.0 / 41.0) * (1 + 1e-5)).all()              # T2 41 ms (config default)
+        ev = T[0, :, 0, 0] == 0
+        signal.append(np.linalg.norm(M[0, ev, 0, :2].astype(np.float64).mean(axis=0)))
+    assert signal[1] < signal[0] < np.exp(-20.0 / 41.0)
