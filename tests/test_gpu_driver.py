@@ -91,11 +91,17 @@ def test_generated_phantoms_and_config_run_end_to_end(engine, tmp_path):
     reference's negative radius), 5 % volume fraction, FoV 600 um on 300^3 voxels,
     dChi 0.11e-6, Y 0.85 and 0.78, seed 45, along B0; a GRE config from the
     `config` generator (TE 20 ms, dt 25 us) with NUMBER_OF_SPINS = 1000; run
-    through sim::handler::execute.  Beyond running: the output layout, and
-    magnitudes bounded by T2 decay with the extravascular signal of the phantom
-    with more deoxygenated blood (Y 0.78) the smaller."""
+    through sim::handler::execute.
+
+    Beyond running, the closed form of this geometry: cylinders along B0 have no
+    field outside and a constant field inside (src/phantom/phantom_cylinder.cpp:
+    262-265), and the default substrates are impermeable.  So every spin keeps
+    |Mxy| = exp(-TE/T2), outside spins keep the excitation's angle, and inside
+    spins are all turned by the same angle: the inside field's phase per step
+    summed over the 801 steps up to the echo (field added before the echo is
+    recorded, src/sim/kernels.cu:171-215), which grows with 1 - Y."""
     import ctypes as C
-    names = []
+    names, f_in = [], []
     for k, Y in enumerate((0.85, 0.78)):
         spec = dict(kind=E.SW_SHAPE_AXONS, label=1, radius_um=-20.0, volume_fraction=5.0, random_orientation=0,
                     orientation_deg=0.0, dchi=0.11e-6, Y=Y, seed=45, trunc_radii=0.0, radius_min_um=0.0)
@@ -103,6 +109,9 @@ def test_generated_phantoms_and_config_run_end_to_end(engine, tmp_path):
         m, f = ph.download(field=True)
         ph.close()
         assert abs(100.0 * np.count_nonzero(m) / m.size - 5.0) < 2.0     # tests/test_phantom.cpp:15-22
+        inside = np.unique(f[m == 1])
+        assert inside.size == 1 and (f[m == 0] == 0).all()
+        f_in.append(inside[0])
         names.append(f"phantom{k + 1}.h5")
         H.write_phantom(tmp_path / names[-1], m, f, np.full(3, np.float32(600e-6)))
     H.generate_config("GRE", 20000, 25, names, tmp_path / "gre.ini")
@@ -112,14 +121,23 @@ def test_generated_phantoms_and_config_run_end_to_end(engine, tmp_path):
     L = H.load()
     assert L.sw_sim_execute(0, 0, files, 1) == E.SW_OK, L.sw_host_last_error().decode()
     c = H.load_config(tmp_path / "gre_sim.ini")
-    signal = []
-    for out in c["outputs"]:
+    assert c["n_timepoints"] == 801 and list(c["TE_t"]) == [800] and list(c["pXY"]) == [1, 0, 0, 1]
+    e2 = np.exp(-20.0 / float(c["T2_ms"][0]))
+    step_deg = [float(np.float32(x) * np.float32(E.tesla2deg(c["B0"], 25))) for x in f_in]
+    for out, sd in zip(c["outputs"], step_deg):
         M = H.h5_read(out, "M")
         T = H.h5_read(out, "T", np.uint8)
         assert M.shape == (1, 1000, 1, 3) and T.shape == (1, 1000, 1, 1)
         np.testing.assert_array_equal(H.h5_read(out, "TE").ravel(), np.float32([20000e-6]))
-        mxy = np.linalg.norm(M[0, :, 0, :2].astype(np.float64), axis=-1)
-        assert (mxy <= np.exp(-20.0 / 41.0) * (1 + 1e-5)).all()              # T2 41 ms (config default)
-        ev = T[0, :, 0, 0] == 0
-        signal.append(np.linalg.norm(M[0, ev, 0, :2].astype(np.float64).mean(axis=0)))
-    assert signal[1] < signal[0] < np.exp(-20.0 / 41.0)
+        mxy = M[0, :, 0, :2].astype(np.float64)
+        np.testing.assert_allclose(np.linalg.norm(mxy, axis=-1), e2, rtol=2e-5)
+        z = mxy[:, 0] + 1j * mxy[:, 1]
+        lab = T[0, :, 0, 0]
+        assert (lab == 0).sum() > 800 and (lab == 1).sum() > 10
+        z0, z1 = z[lab == 0], z[lab == 1]
+        for zz in (z0, z1):                                 # one angle per label, to float rounding
+            assert np.degrees(np.abs(np.angle(zz / zz.mean())).max()) < 1e-3
+        d = np.degrees(np.angle(z1.mean() / z0.mean())) % 360.0
+        want = (801 * sd) % 360.0
+        assert min(abs(d - want), abs((360.0 - want) - d)) < 0.05, (d, want)      # either sense of rotation
+    assert step_deg[1] / step_deg[0] == pytest.approx((1 - 0.78) / (1 - 0.85), rel=1e-5)
