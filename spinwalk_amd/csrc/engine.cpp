@@ -647,19 +647,15 @@ namespace {
 // Writes the visiting order of a's spins into out[0, n_local).
 int build_order_into(sw_engine *e, const sw::WalkArgs &a, uint32_t *out, hipStream_t s) {
     const uint32_t n = a.n_local;
-    const size_t tmp = sw::sort_pairs_temp_bytes(n);
-    const size_t need = 3 * (size_t)n * sizeof(uint32_t) + tmp + 256;
-    if (need > e->sort_bytes) {
+    const sw::OrderPlan plan = sw::order_plan(n, a.nx, a.ny, a.nz);
+    if (plan.total_bytes > e->sort_bytes) {
         SW_HIP(e, hipStreamSynchronize(s));
         dev_free(e->d_sort);
         e->d_sort = nullptr;
-        SW_HIP(e, hipMalloc(&e->d_sort, need));
-        e->sort_bytes = need;
+        SW_HIP(e, hipMalloc(&e->d_sort, plan.total_bytes));
+        e->sort_bytes = plan.total_bytes;
     }
-    uint32_t *kin = (uint32_t *)e->d_sort, *kout = kin + n, *vin = kout + n;
-    void *t = (void *)(((uintptr_t)(vin + n) + 255) & ~(uintptr_t)255);
-    SW_HIP(e, sw::launch_brick_keys(a.XYZ0, a.xyz_scale, n, a.s2g, a.nx, a.ny, a.nz, kin, vin, s));
-    SW_HIP(e, sw::launch_sort_pairs(t, tmp, kin, kout, vin, out, n, s));
+    SW_HIP(e, sw::launch_spatial_order(plan, e->d_sort, a.XYZ0, a.xyz_scale, n, a.s2g, a.nx, a.ny, a.nz, out, s));
     return SW_OK;
 }
 

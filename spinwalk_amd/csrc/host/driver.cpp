@@ -150,8 +150,10 @@ int run_phantom(sw_engine *eng, swhost::Config &c, size_t iph, std::vector<float
     const size_t need = 24 * (size_t)n + 4 * m1_slice * n_sc + t_slice * n_sc +
                         (streamed ? 2 * 4 * xc : 4 * x_slice * n_sc);
     if (need > fr) return fail(SW_E_NOMEM, "not enough device memory for the spin buffers");
+    const bool x_failed = streamed ? (dXa.alloc(4 * xc) != hipSuccess || dXb.alloc(4 * xc) != hipSuccess)
+                                   : dX1.alloc(XYZ1.size() * 4) != hipSuccess;
     if (dM0.alloc(M0.size() * 4) || dX0.alloc(XYZ0.size() * 4) || dM1.alloc(M1.size() * 4) || dT.alloc(T.size()) ||
-        (streamed ? (dXa.alloc(4 * xc) || dXb.alloc(4 * xc)) : dX1.alloc(XYZ1.size() * 4)))
+        x_failed)
         return fail(SW_E_HIP, "hipMalloc failed");
     // the device outputs start from the host buffers (the reference's d_M1 = M1)
     if (hipMemcpy(dM0.p, M0.data(), M0.size() * 4, hipMemcpyHostToDevice) ||

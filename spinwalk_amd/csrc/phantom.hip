@@ -74,32 +74,6 @@ __global__ void records_kernel(const uint8_t *mask, const float *field, uint64_t
     }
 }
 
-__global__ void brick_keys_kernel(const float *XYZ0, float xyz_scale, uint32_t n, double s0, double s1,
-                                  double s2, int64_t nx, int64_t ny, int64_t nz, uint32_t *keys,
-                                  uint32_t *vals) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double x = (double)(XYZ0[3 * (uint64_t)i + 0] * xyz_scale) * s0;
-    const double y = (double)(XYZ0[3 * (uint64_t)i + 1] * xyz_scale) * s1;
-    const double z = (double)(XYZ0[3 * (uint64_t)i + 2] * xyz_scale) * s2;
-    // 8^3-voxel bricks in Morton order (10 bits per axis covers 8192 voxels)
-    auto clampb = [](double v, int64_t nmax) -> uint32_t {
-        int64_t b = (int64_t)v;
-        if (b < 0) b = 0;
-        if (b >= nmax) b = nmax - 1;
-        return (uint32_t)(b >> 3) & 1023u;
-    };
-    const uint32_t bx = clampb(x, nx), by = clampb(y, ny), bz = clampb(z, nz);
-    uint32_t k = 0;
-    for (int j = 0; j < 10; j++) {
-        k |= ((bz >> j) & 1u) << (3 * j);
-        k |= ((by >> j) & 1u) << (3 * j + 1);
-        k |= ((bx >> j) & 1u) << (3 * j + 2);
-    }
-    keys[i] = k;
-    vals[i] = i;
-}
-
 }  // namespace
 
 hipError_t launch_phantom_stats(const uint8_t *mask, const float *field, uint64_t n, uint32_t *stats,
@@ -144,14 +118,6 @@ hipError_t launch_records(const uint8_t *mask, const float *field, uint64_t nx, 
             hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 3>), g, b, 0, s, mask, field, n,
                                (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
     }
-    return hipGetLastError();
-}
-
-hipError_t launch_brick_keys(const float *XYZ0, float xyz_scale, uint32_t n, const double *s2g, int64_t nx,
-                             int64_t ny, int64_t nz, uint32_t *keys, uint32_t *vals, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(brick_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, s, XYZ0, xyz_scale, n,
-                       s2g[0], s2g[1], s2g[2], nx, ny, nz, keys, vals);
     return hipGetLastError();
 }
 

@@ -5,7 +5,7 @@
 // (demo/spinwalk_bold.ipynb:733-736).  Here the sums of Mx, My and the counts
 // are formed per (echo, label) in float64 on the device.  Pass 1: block j sums
 // a fixed chunk of spins for one (echo, label) pair into partial[pair][j];
-// pass 2 adds the partials of each pair in chunk order.  The order of additions
+// pass 2 adds the partials of each pair in a fixed order (one block per pair).  The order of additions
 // is fixed by the launch shape alone, so results are bit-reproducible.
 #include <hip/hip_runtime.h>
 
@@ -50,15 +50,30 @@ __global__ void reduce_pass1(const R *M1, const uint8_t *T, uint32_t n_spins, ui
     }
 }
 
+// One block per (echo, label) pair: thread t adds the partials t, t+256, ... in
+// order, then the block's tree adds the 256 thread sums -- the same fixed order
+// on every launch.
 __global__ void reduce_pass2(const double *partial, uint32_t n_pairs, uint32_t n_chunks, double *out) {
-    const uint32_t pair = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pair >= n_pairs) return;
+    const uint32_t pair = blockIdx.x;
     double sx = 0, sy = 0, sc = 0;
-    for (uint32_t c = 0; c < n_chunks; c++) {
+    for (uint32_t c = threadIdx.x; c < n_chunks; c += kThreads) {
         const double *p = partial + 3 * ((uint64_t)pair * n_chunks + c);
         sx += p[0]; sy += p[1]; sc += p[2];
     }
-    out[3 * pair + 0] = sx; out[3 * pair + 1] = sy; out[3 * pair + 2] = sc;
+    __shared__ double sh[3][kThreads];
+    sh[0][threadIdx.x] = sx; sh[1][threadIdx.x] = sy; sh[2][threadIdx.x] = sc;
+    __syncthreads();
+    for (uint32_t w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+            sh[2][threadIdx.x] += sh[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[3 * pair + 0] = sh[0][0]; out[3 * pair + 1] = sh[1][0]; out[3 * pair + 2] = sh[2][0];
+    }
 }
 
 }  // namespace
@@ -75,7 +90,7 @@ hipError_t launch_echo_reduce(const void *M1, int f64, const uint8_t *T, uint32_
     uint32_t n_chunks = (n_spins + kChunk - 1) / kChunk;
     if (n_chunks == 0) {
         // no spins: sums are zero
-        hipLaunchKernelGGL(reduce_pass2, dim3((n_pairs + 63) / 64), dim3(64), 0, s, partial, n_pairs, 0u, out);
+        hipLaunchKernelGGL(reduce_pass2, dim3(n_pairs), dim3(kThreads), 0, s, partial, n_pairs, 0u, out);
         return hipGetLastError();
     }
     const dim3 g1(n_chunks, n_pairs);
@@ -87,7 +102,7 @@ hipError_t launch_echo_reduce(const void *M1, int f64, const uint8_t *T, uint32_
                            n_te, n_sub, n_chunks, partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(reduce_pass2, dim3((n_pairs + 63) / 64), dim3(64), 0, s, partial, n_pairs, n_chunks, out);
+    hipLaunchKernelGGL(reduce_pass2, dim3(n_pairs), dim3(kThreads), 0, s, partial, n_pairs, n_chunks, out);
     return hipGetLastError();
 }
 

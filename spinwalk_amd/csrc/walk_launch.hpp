@@ -54,14 +54,19 @@ hipError_t launch_echo_reduce(const void *M1, int f64, const uint8_t *T, uint32_
 // Clears n counters with a fill kernel (no memset node, safe under graph capture).
 hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
 
-// Radix sort of (key, value) pairs (hipCUB); temp of sort_pairs_temp_bytes(n).
-size_t sort_pairs_temp_bytes(uint32_t n);
-hipError_t launch_sort_pairs(void *temp, size_t temp_bytes, const uint32_t *kin, uint32_t *kout,
-                             const uint32_t *vin, uint32_t *vout, uint32_t n, hipStream_t s);
-
-// Spatial visiting order: keys = brick id of each spin's start voxel.
-hipError_t launch_brick_keys(const float *XYZ0, float xyz_scale, uint32_t n, const double *s2g,
-                             int64_t nx, int64_t ny, int64_t nz, uint32_t *keys, uint32_t *vals,
-                             hipStream_t s);
+// The spatial visiting order (sort.hip): order[k] = the spin walked k-th, sorted
+// by the Morton index of its start voxel's 8^3 brick.  order_plan sizes the
+// scratch (total_bytes) and picks a counting sort or a radix sort.
+struct OrderPlan {
+    bool counting;
+    uint32_t bits;          // Morton bits per axis
+    uint32_t bins;          // counting sort: 2^(3 bits)
+    size_t temp_bytes;      // hipCUB scan or sort temp
+    size_t total_bytes;     // scratch needed
+};
+OrderPlan order_plan(uint32_t n, int64_t nx, int64_t ny, int64_t nz);
+hipError_t launch_spatial_order(const OrderPlan &p, void *scratch, const float *XYZ0, float xyz_scale, uint32_t n,
+                                const double *s2g, int64_t nx, int64_t ny, int64_t nz, uint32_t *order,
+                                hipStream_t s);
 
 }  // namespace sw
