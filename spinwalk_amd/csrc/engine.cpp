@@ -645,6 +645,11 @@ namespace {
 // neighbouring voxels, and the XCD-aware block mapping of the walk kernel gives
 // each XCD (own L2) a compact region of the phantom.
 // Writes the visiting order of a's spins into out[0, n_local).
+// Bands per tile part in the batched schedule (measured on the FoV sweep, one
+// phantom: 1 -> 170.8 ms, 16 -> 153.8, 64 -> 150.7; on 64 phantoms: 1 -> 163.5,
+// 16 -> 151.2, 64 -> 155.2).
+constexpr uint32_t kBandsShared = 64, kBandsMixed = 16;
+
 int build_order_into(sw_engine *e, const sw::WalkArgs &a, uint32_t *out, hipStream_t s) {
     const uint32_t n = a.n_local;
     const sw::OrderPlan plan = sw::order_plan(n, a.nx, a.ny, a.nz);
@@ -926,12 +931,52 @@ SW_API int sw_batch_create(sw_engine *e, const sw_batch_item *items, uint32_t n_
         if (idx.empty()) continue;
         auto cost = [&](uint32_t i) { return (uint64_t)(args[i].n_dummy + 1) * args[i].n_timepoints; };
         std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return cost(x) > cost(y); });
-        std::vector<sw::BatchChunk> lists[8];
-        for (uint32_t i : idx) {
+        // Within an XCD's list, tiles of one cost class (a factor of two in steps
+        // per spin) take turns band by band through their parts, longer classes
+        // first.  The chunks resident on an XCD then start in one band of each
+        // tile's part: a narrow window of its phantom, whose lines stay in L2, and
+        // the same window for tiles that share a phantom.  Bands are finer (64)
+        // when every tile of the class walks one phantom, since their lines are
+        // shared; 16 otherwise.
+        auto cls_of = [&](uint32_t i) {
+            uint32_t c = 0;
+            for (uint64_t v = cost(i); v > 1; v >>= 1) c++;
+            return c;
+        };
+        std::vector<std::pair<uint32_t, const void *>> cls_rec;   // (class, phantom records) per tile
+        for (uint32_t i : idx) cls_rec.push_back({cls_of(i), args[i].rec});
+        auto bands_of = [&](uint32_t cls) {
+            const void *rec = nullptr;
+            for (const auto &q : cls_rec) {
+                if (q.first != cls) continue;
+                if (rec && q.second != rec) return kBandsMixed;
+                rec = q.second;
+            }
+            return kBandsShared;
+        };
+        struct Key { uint32_t cls, band, rank, item, c; };
+        std::vector<Key> keys[8];
+        for (uint32_t r = 0; r < (uint32_t)idx.size(); r++) {
+            const uint32_t i = idx[r];
+            const uint32_t cls = cls_of(i);
+            const uint32_t bands = bands_of(cls);
             const uint32_t nc = (args[i].n_local + 255) / 256;
-            for (uint32_t k = 0; k < 8; k++)
-                for (uint32_t c = (uint32_t)((uint64_t)nc * k / 8); c < (uint32_t)((uint64_t)nc * (k + 1) / 8); c++)
-                    lists[k].push_back({i, c * 256u});
+            for (uint32_t k = 0; k < 8; k++) {
+                const uint32_t c0 = (uint32_t)((uint64_t)nc * k / 8), c1 = (uint32_t)((uint64_t)nc * (k + 1) / 8);
+                for (uint32_t c = c0; c < c1; c++) {
+                    const uint32_t band = (uint32_t)((uint64_t)(c - c0) * bands / (c1 - c0));
+                    keys[k].push_back({cls, band, r, i, c});
+                }
+            }
+        }
+        std::vector<sw::BatchChunk> lists[8];
+        for (uint32_t k = 0; k < 8; k++) {
+            std::stable_sort(keys[k].begin(), keys[k].end(), [](const Key &x, const Key &y) {
+                if (x.cls != y.cls) return x.cls > y.cls;
+                if (x.band != y.band) return x.band < y.band;
+                return x.rank < y.rank;
+            });
+            for (const Key &q : keys[k]) lists[k].push_back({q.item, q.c * 256u});
         }
         size_t per = 0;
         for (auto &l : lists) per = std::max(per, l.size());
