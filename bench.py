@@ -132,6 +132,11 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
 
     b, items = make_batch(prec, fmt)
     st = {}
+    tuning = None
+    if args.tune:
+        tuning = b.tune(stream=stream)
+        log(f"[bench] autotuner: {tuning['schedule']}, {tuning['best_ms']:.2f} ms per run "
+            f"(starting schedule {tuning['first_ms']:.2f} ms)")
 
     def batched():
         st.update(b.run(stream=stream, stats=True)["total"])
@@ -177,7 +182,7 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
                    if "phantom" in wl else f"{len(phs)} phantoms of cylinders, spheres and packed axons",
                    "tiles": len(tiles), "spins_per_tile": tiles[0]["n_local"], "spin_steps": total_steps,
                    "precision": args.precision, "field": args.field, "mode": "batched, one HIP graph",
-                   "rng": "philox4x32-10"},
+                   "schedule": b.schedule, "autotuned": tuning is not None, "rng": "philox4x32-10"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_spin_step": bpe,
                      "kernel_ms": st["kernel_ms"]},
@@ -186,6 +191,8 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
         "signal_error_vs_f64": {"max_rel": err_b, "max_abs": max_abs, "worst": worst,
                                 "per_tile": per_tile if len(tiles) <= 8 else None},
     }
+    if tuning is not None:
+        out["autotuner"] = tuning
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"[bench] CPU baseline: oracle on {threads} threads, {args.cpu_seconds} s budget, first tile")
@@ -251,6 +258,8 @@ def main():
                     help="after the timed region, time one host-buffer call (sw_walk_host: uploads, walk, downloads)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed region, run one device copy of this size (profiler calibration)")
+    ap.add_argument("--tune", action="store_true",
+                    help="tile workloads: run the batch autotuner (sw_batch_tune) before the timed region")
     ap.add_argument("--probe", action="store_true",
                     help="instead of the walk: the memory ceilings of the gather (random 128-byte line reads at "
                          "several working-set sizes) and of a streaming copy")

@@ -249,6 +249,28 @@ SW_API int  sw_batch_create(sw_engine *eng, const sw_batch_item *items, uint32_t
 SW_API int  sw_batch_run(sw_batch *b, void *stream, sw_walk_stats *per_item, sw_walk_stats *total);
 SW_API void sw_batch_destroy(sw_batch *b);
 
+/* ---- batch schedule and its autotuner (no counterpart in the reference) -----
+ * The order in which a batch's 256-spin chunks are walked.  Every tile is split
+ * into 8 parts, one per XCD; within an XCD, tiles of one cost class (a factor
+ * of two in steps per spin) take turns in bands through their parts, longer
+ * classes first.  A schedule changes only that order: results are the same bit
+ * for bit under every schedule. */
+typedef struct sw_batch_schedule {
+    uint32_t chunk_spins;    /* spins per chunk = workgroup size: 64, 128 or 256        */
+    uint32_t bands_shared;   /* bands per part when every tile of a class walks one phantom */
+    uint32_t bands_mixed;    /* bands per part for the other classes                   */
+} sw_batch_schedule;
+SW_API int  sw_batch_get_schedule(const sw_batch *b, sw_batch_schedule *out);
+/* Rebuilds the chunk table and the graph for `s` (a tuned schedule, say). */
+SW_API int  sw_batch_set_schedule(sw_batch *b, const sw_batch_schedule *s);
+/* Times full runs of the batch on `stream` under candidate schedules (chunk size,
+ * then the band counts of the class kinds present), keeps the fastest and
+ * recaptures the graph.  At most max_trials timed runs after two of the current
+ * schedule (0: no limit).  *best_ms / *first_ms: the kept schedule's run and the
+ * starting schedule's. */
+SW_API int  sw_batch_tune(sw_batch *b, void *stream, uint32_t max_trials, sw_batch_schedule *best,
+                          float *best_ms, float *first_ms);
+
 /* ---- echo reduction (demo/spinwalk_bold.ipynb:733-736), FP64 ----------------
  * out[(te*n_sub + label)*3 + {0,1,2}] = {sum Mx, sum My, count} over spins with
  * T[spin][te] == label, summed in float64 in a fixed order (deterministic). */

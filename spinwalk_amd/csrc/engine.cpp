@@ -645,11 +645,6 @@ namespace {
 // neighbouring voxels, and the XCD-aware block mapping of the walk kernel gives
 // each XCD (own L2) a compact region of the phantom.
 // Writes the visiting order of a's spins into out[0, n_local).
-// Bands per tile part in the batched schedule (measured on the FoV sweep, one
-// phantom: 1 -> 170.8 ms, 16 -> 153.8, 64 -> 150.7; on 64 phantoms: 1 -> 163.5,
-// 16 -> 151.2, 64 -> 155.2).
-constexpr uint32_t kBandsShared = 64, kBandsMixed = 16;
-
 int build_order_into(sw_engine *e, const sw::WalkArgs &a, uint32_t *out, hipStream_t s) {
     const uint32_t n = a.n_local;
     const sw::OrderPlan plan = sw::order_plan(n, a.nx, a.ny, a.nz);
@@ -813,6 +808,12 @@ struct sw_batch {
     int rng = 0, field = 0, prec = 0;
     bool idx32 = false;
     size_t lds = 0;
+    // what the chunk table is built from: per tile, its CROSS_FOV group, spins,
+    // steps per spin and phantom records
+    struct Tile { bool cross; uint32_t n_local; uint64_t cost; const void *rec; };
+    std::vector<Tile> tiles;
+    sw_batch_schedule sched{};
+    size_t chunks_cap = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -825,8 +826,8 @@ hipError_t batch_enqueue(const sw_batch *b, hipStream_t s) {
     hipError_t err = sw::launch_zero_u64(b->d_counters, (int)(4 * b->n_items), s);
     if (err != hipSuccess) return err;
     for (const auto &g : b->groups) {
-        err = sw::launch_walk_batch(b->d_items, b->d_chunks + g.offset, g.n_slots, b->rng, b->field, b->prec, g.cross,
-                                    b->idx32, b->lds, s);
+        err = sw::launch_walk_batch(b->d_items, b->d_chunks + g.offset, g.n_slots, b->sched.chunk_spins, b->rng,
+                                    b->field, b->prec, g.cross, b->idx32, b->lds, s);
         if (err != hipSuccess) return err;
     }
     for (const auto &r : b->reduces) {
@@ -834,6 +835,146 @@ hipError_t batch_enqueue(const sw_batch *b, hipStream_t s) {
         if (err != hipSuccess) return err;
     }
     return hipSuccess;
+}
+
+// The default batched schedule: chunks of 256 spins; tiles of one phantom
+// interleave chunk by chunk in relative position (2^20 bands: finer than any
+// part), so tiles that walk the same trajectories (phase-cycling or gradient
+// scales of one seed) run the same spins side by side and share every line;
+// tiles of different phantoms take turns in 16 bands.  Measured: the 1000^3
+// bSSFP batch (4 tiles, one phantom) 2.676 s tile after tile, 2.511 s with 64
+// bands, 2.029 s chunk by chunk; the 64-phantom batch 163.5 ms tile after tile,
+// 151.2 ms with 16 bands, 155.2 ms with 64.  sw_batch_tune searches around it.
+constexpr sw_batch_schedule kDefaultSchedule = {256u, 1u << 20, 16u};
+
+// The chunk table of a schedule: per CROSS_FOV group, each tile's chunks of
+// sc.chunk_spins spins split into 8 contiguous parts (one per XCD; a part is one
+// spatial region of the tile, since chunks follow the visiting order).  Within
+// an XCD's list, tiles of one cost class (a factor of two in steps per spin)
+// take turns band by band through their parts, longer classes first.  The
+// chunks resident on an XCD then start in one band of each tile's part: a
+// narrow window of its phantom, whose lines stay in L2, and the same window for
+// tiles that share a phantom.  A class whose tiles all walk one phantom uses
+// sc.bands_shared bands per part, any other class sc.bands_mixed.  *kinds
+// receives bit 0 if some class is shared, bit 1 if some class is mixed.
+void build_chunk_table(const sw_batch &b, const sw_batch_schedule &sc, std::vector<sw::BatchChunk> &table,
+                       std::vector<sw_batch::Group> &groups, unsigned *kinds) {
+    table.clear();
+    groups.clear();
+    *kinds = 0;
+    const uint32_t cs = sc.chunk_spins;
+    for (int cross = 0; cross < 2; cross++) {
+        std::vector<uint32_t> idx;
+        for (uint32_t i = 0; i < (uint32_t)b.tiles.size(); i++)
+            if (b.tiles[i].cross == (cross != 0) && b.tiles[i].n_local) idx.push_back(i);
+        if (idx.empty()) continue;
+        auto cost = [&](uint32_t i) { return b.tiles[i].cost; };
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return cost(x) > cost(y); });
+        auto cls_of = [&](uint32_t i) {
+            uint32_t c = 0;
+            for (uint64_t v = cost(i); v > 1; v >>= 1) c++;
+            return c;
+        };
+        auto shared = [&](uint32_t cls) {
+            const void *rec = nullptr;
+            for (uint32_t i : idx) {
+                if (cls_of(i) != cls) continue;
+                if (rec && b.tiles[i].rec != rec) return false;
+                rec = b.tiles[i].rec;
+            }
+            return true;
+        };
+        struct Key { uint32_t cls, band, rank, item, c; };
+        std::vector<Key> keys[8];
+        for (uint32_t r = 0; r < (uint32_t)idx.size(); r++) {
+            const uint32_t i = idx[r];
+            const uint32_t cls = cls_of(i);
+            const bool sh = shared(cls);
+            *kinds |= sh ? 1u : 2u;
+            const uint32_t bands = sh ? sc.bands_shared : sc.bands_mixed;
+            const uint32_t nc = (b.tiles[i].n_local + cs - 1) / cs;
+            for (uint32_t k = 0; k < 8; k++) {
+                const uint32_t c0 = (uint32_t)((uint64_t)nc * k / 8), c1 = (uint32_t)((uint64_t)nc * (k + 1) / 8);
+                for (uint32_t c = c0; c < c1; c++) {
+                    const uint32_t band = (uint32_t)((uint64_t)(c - c0) * bands / (c1 - c0));
+                    keys[k].push_back({cls, band, r, i, c});
+                }
+            }
+        }
+        std::vector<sw::BatchChunk> lists[8];
+        size_t per = 0;
+        for (uint32_t k = 0; k < 8; k++) {
+            std::stable_sort(keys[k].begin(), keys[k].end(), [](const Key &x, const Key &y) {
+                if (x.cls != y.cls) return x.cls > y.cls;
+                if (x.band != y.band) return x.band < y.band;
+                return x.rank < y.rank;
+            });
+            for (const Key &q : keys[k]) lists[k].push_back({q.item, q.c * cs});
+            per = std::max(per, lists[k].size());
+        }
+        sw_batch::Group g;
+        g.cross = cross != 0;
+        g.offset = (uint32_t)table.size();
+        g.n_slots = (uint32_t)(8 * per);
+        for (auto &l : lists) {
+            l.resize(per, sw::BatchChunk{sw::kNoItem, 0});
+            table.insert(table.end(), l.begin(), l.end());
+        }
+        groups.push_back(g);
+    }
+}
+
+bool schedule_ok(const sw_batch_schedule &sc) {
+    return (sc.chunk_spins == 64 || sc.chunk_spins == 128 || sc.chunk_spins == 256) && sc.bands_shared >= 1 &&
+           sc.bands_mixed >= 1;
+}
+
+// Makes sc the batch's schedule: builds and uploads its chunk table and, when
+// `capture`, captures the batch as one HIP graph (counter reset, walks, echo
+// sums).  Without capture the batch runs by plain launches until the next call.
+int apply_schedule(sw_engine *e, sw_batch *b, const sw_batch_schedule &sc, hipStream_t s, bool capture) {
+    std::vector<sw::BatchChunk> table;
+    std::vector<sw_batch::Group> groups;
+    unsigned kinds = 0;
+    build_chunk_table(*b, sc, table, groups, &kinds);
+    SW_HIP(e, hipStreamSynchronize(s));          // the previous table may be in use
+    if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    if (table.size() > b->chunks_cap) {
+        dev_free(b->d_chunks);
+        b->d_chunks = nullptr;
+        b->chunks_cap = 0;
+        SW_HIP(e, hipMalloc((void **)&b->d_chunks, table.size() * sizeof(sw::BatchChunk)));
+        b->chunks_cap = table.size();
+    }
+    if (!table.empty())
+        SW_HIP(e, hipMemcpyAsync(b->d_chunks, table.data(), table.size() * sizeof(sw::BatchChunk),
+                                 hipMemcpyHostToDevice, s));
+    SW_HIP(e, hipStreamSynchronize(s));          // the host table goes out of scope
+    b->groups = groups;
+    b->sched = sc;
+    if (!capture) return SW_OK;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const hipError_t le = batch_enqueue(b, s);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(s, &g);
+        if (le == hipSuccess && ce == hipSuccess && g) {
+            b->graph = g;
+            if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) b->exec = nullptr;
+        } else if (g) {
+            (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();
+    }
+    return SW_OK;
+}
+
+unsigned schedule_kinds(const sw_batch &b) {
+    std::vector<sw::BatchChunk> table;
+    std::vector<sw_batch::Group> groups;
+    unsigned kinds = 0;
+    build_chunk_table(b, b.sched, table, groups, &kinds);
+    return kinds;
 }
 
 void batch_free(sw_batch *b) {
@@ -921,93 +1062,14 @@ SW_API int sw_batch_create(sw_engine *e, const sw_batch_item *items, uint32_t n_
     }
     SW_HIP(e, dev_upload(&b->d_items, args.data(), args.size(), s));
 
-    // ---- chunk table: per CROSS_FOV group, each tile's 256-spin chunks split into
-    // 8 contiguous parts (one per XCD), tiles in order of decreasing steps per spin
-    std::vector<sw::BatchChunk> table;
-    for (int cross = 0; cross < 2; cross++) {
-        std::vector<uint32_t> idx;
-        for (uint32_t i = 0; i < n_items; i++)
-            if (args[i].cross_fov == cross && args[i].n_local) idx.push_back(i);
-        if (idx.empty()) continue;
-        auto cost = [&](uint32_t i) { return (uint64_t)(args[i].n_dummy + 1) * args[i].n_timepoints; };
-        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return cost(x) > cost(y); });
-        // Within an XCD's list, tiles of one cost class (a factor of two in steps
-        // per spin) take turns band by band through their parts, longer classes
-        // first.  The chunks resident on an XCD then start in one band of each
-        // tile's part: a narrow window of its phantom, whose lines stay in L2, and
-        // the same window for tiles that share a phantom.  Bands are finer (64)
-        // when every tile of the class walks one phantom, since their lines are
-        // shared; 16 otherwise.
-        auto cls_of = [&](uint32_t i) {
-            uint32_t c = 0;
-            for (uint64_t v = cost(i); v > 1; v >>= 1) c++;
-            return c;
-        };
-        std::vector<std::pair<uint32_t, const void *>> cls_rec;   // (class, phantom records) per tile
-        for (uint32_t i : idx) cls_rec.push_back({cls_of(i), args[i].rec});
-        auto bands_of = [&](uint32_t cls) {
-            const void *rec = nullptr;
-            for (const auto &q : cls_rec) {
-                if (q.first != cls) continue;
-                if (rec && q.second != rec) return kBandsMixed;
-                rec = q.second;
-            }
-            return kBandsShared;
-        };
-        struct Key { uint32_t cls, band, rank, item, c; };
-        std::vector<Key> keys[8];
-        for (uint32_t r = 0; r < (uint32_t)idx.size(); r++) {
-            const uint32_t i = idx[r];
-            const uint32_t cls = cls_of(i);
-            const uint32_t bands = bands_of(cls);
-            const uint32_t nc = (args[i].n_local + 255) / 256;
-            for (uint32_t k = 0; k < 8; k++) {
-                const uint32_t c0 = (uint32_t)((uint64_t)nc * k / 8), c1 = (uint32_t)((uint64_t)nc * (k + 1) / 8);
-                for (uint32_t c = c0; c < c1; c++) {
-                    const uint32_t band = (uint32_t)((uint64_t)(c - c0) * bands / (c1 - c0));
-                    keys[k].push_back({cls, band, r, i, c});
-                }
-            }
-        }
-        std::vector<sw::BatchChunk> lists[8];
-        for (uint32_t k = 0; k < 8; k++) {
-            std::stable_sort(keys[k].begin(), keys[k].end(), [](const Key &x, const Key &y) {
-                if (x.cls != y.cls) return x.cls > y.cls;
-                if (x.band != y.band) return x.band < y.band;
-                return x.rank < y.rank;
-            });
-            for (const Key &q : keys[k]) lists[k].push_back({q.item, q.c * 256u});
-        }
-        size_t per = 0;
-        for (auto &l : lists) per = std::max(per, l.size());
-        sw_batch::Group g;
-        g.cross = cross != 0;
-        g.offset = (uint32_t)table.size();
-        g.n_slots = (uint32_t)(8 * per);
-        for (auto &l : lists) {
-            l.resize(per, sw::BatchChunk{sw::kNoItem, 0});
-            table.insert(table.end(), l.begin(), l.end());
-        }
-        b->groups.push_back(g);
-    }
-    SW_HIP(e, dev_upload(&b->d_chunks, table.data(), table.size(), s));
+    for (uint32_t i = 0; i < n_items; i++)
+        b->tiles.push_back({args[i].cross_fov != 0, args[i].n_local,
+                            (uint64_t)(args[i].n_dummy + 1) * args[i].n_timepoints, args[i].rec});
     SW_HIP(e, hipStreamSynchronize(s));        // host tables go out of scope
     SW_HIP(e, hipEventCreate(&b->ev0));
     SW_HIP(e, hipEventCreate(&b->ev1));
-
-    // ---- one graph: counter reset, walks, echo sums
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-        const hipError_t le = batch_enqueue(b.get(), s);
-        hipGraph_t g = nullptr;
-        const hipError_t ce = hipStreamEndCapture(s, &g);
-        if (le == hipSuccess && ce == hipSuccess && g) {
-            b->graph = g;
-            if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) b->exec = nullptr;
-        } else if (g) {
-            (void)hipGraphDestroy(g);
-        }
-        (void)hipGetLastError();
-    }
+    int rc = apply_schedule(e, b.get(), kDefaultSchedule, s, true);
+    if (rc) return rc;
     *out = b.release();
     return SW_OK;
 }
@@ -1046,6 +1108,85 @@ SW_API int sw_batch_run(sw_batch *b, void *stream, sw_walk_stats *per_item, sw_w
 }
 
 SW_API void sw_batch_destroy(sw_batch *b) { batch_free(b); }
+
+SW_API int sw_batch_get_schedule(const sw_batch *b, sw_batch_schedule *out) {
+    if (!b || !out) return SW_E_INVALID;
+    *out = b->sched;
+    return SW_OK;
+}
+
+SW_API int sw_batch_set_schedule(sw_batch *b, const sw_batch_schedule *sc) {
+    if (!b) return SW_E_INVALID;
+    sw_engine *e = b->eng;
+    if (!sc || !schedule_ok(*sc))
+        return fail(e, SW_E_INVALID, "schedule needs chunk_spins 64, 128 or 256 and at least one band");
+    SW_HIP(e, hipSetDevice(b->device));
+    return apply_schedule(e, b, *sc, e->stream, true);
+}
+
+// Coordinate search over the schedule: starting from the current one, each
+// parameter in turn (chunk size, then the band counts of the class kinds the
+// batch has) takes the candidate value that times fastest, a candidate having to
+// beat the best by 1 %.  Each trial is one full run of the batch, timed with HIP
+// events; schedules change only the order in which chunks are walked, so every
+// trial leaves the batch's outputs as any run does.
+SW_API int sw_batch_tune(sw_batch *b, void *stream, uint32_t max_trials, sw_batch_schedule *best_out,
+                         float *best_ms, float *first_ms) {
+    if (!b) return SW_E_INVALID;
+    sw_engine *e = b->eng;
+    SW_HIP(e, hipSetDevice(b->device));
+    hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    uint32_t trials = 0;
+    auto timed = [&](const sw_batch_schedule &sc, float *ms) -> int {
+        int rc = apply_schedule(e, b, sc, s, false);
+        if (rc) return rc;
+        SW_HIP(e, hipEventRecord(b->ev0, s));
+        SW_HIP(e, batch_enqueue(b, s));
+        SW_HIP(e, hipEventRecord(b->ev1, s));
+        SW_HIP(e, hipEventSynchronize(b->ev1));
+        SW_HIP(e, hipEventElapsedTime(ms, b->ev0, b->ev1));
+        trials++;
+        return SW_OK;
+    };
+    sw_batch_schedule best = b->sched;
+    float t_best = 0, t = 0;
+    int rc = timed(best, &t_best);                 // warm-up: caches, code objects
+    if (rc) return rc;
+    rc = timed(best, &t_best);
+    if (rc) return rc;
+    const float t_first = t_best;
+    const unsigned kinds = schedule_kinds(*b);
+    auto consider = [&](sw_batch_schedule cand) -> int {
+        if (max_trials && trials >= max_trials) return SW_OK;
+        if (cand.chunk_spins == best.chunk_spins && cand.bands_shared == best.bands_shared &&
+            cand.bands_mixed == best.bands_mixed)
+            return SW_OK;
+        int r = timed(cand, &t);
+        if (r) return r;
+        if (t < 0.99f * t_best) { best = cand; t_best = t; }
+        return SW_OK;
+    };
+    for (uint32_t c : {256u, 128u, 64u}) {
+        sw_batch_schedule x = best; x.chunk_spins = c;
+        if ((rc = consider(x))) return rc;
+    }
+    if (kinds & 1u)
+        for (uint32_t v : {1u << 20, 64u, 16u, 4u}) {
+            sw_batch_schedule x = best; x.bands_shared = v;
+            if ((rc = consider(x))) return rc;
+        }
+    if (kinds & 2u)
+        for (uint32_t v : {16u, 64u, 4u, 1u}) {
+            sw_batch_schedule x = best; x.bands_mixed = v;
+            if ((rc = consider(x))) return rc;
+        }
+    rc = apply_schedule(e, b, best, s, true);
+    if (rc) return rc;
+    if (best_out) *best_out = best;
+    if (best_ms) *best_ms = t_best;
+    if (first_ms) *first_ms = t_first;
+    return SW_OK;
+}
 
 SW_API int sw_walk_host(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, const sw_params *p,
                         const sw_walk_opts *o, const float *M0, const float *XYZ0, void *M1, float *XYZ1,

@@ -615,9 +615,9 @@ hipError_t launch_fp(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
 }
 
 template <class Rng, class F, class P>
-hipError_t launch_batch_fp(const WalkArgs *items, const BatchChunk *chunks, uint32_t n_slots, bool cross,
-                           bool idx32, size_t lds, hipStream_t s) {
-    const dim3 grid(n_slots), block(256);
+hipError_t launch_batch_fp(const WalkArgs *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
+                           bool cross, bool idx32, size_t lds, hipStream_t s) {
+    const dim3 grid(n_slots), block(threads);
     if (cross) {
         if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, true>), grid, block, lds, s, items, chunks);
         else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, false>), grid, block, 0, s, items, chunks);
@@ -629,21 +629,21 @@ hipError_t launch_batch_fp(const WalkArgs *items, const BatchChunk *chunks, uint
 }
 
 template <class Rng>
-hipError_t launch_batch_rng(const WalkArgs *items, const BatchChunk *chunks, uint32_t n_slots, int field, int prec,
-                            bool cross, bool idx32, size_t lds, hipStream_t s) {
+hipError_t launch_batch_rng(const WalkArgs *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
+                            int field, int prec, bool cross, bool idx32, size_t lds, hipStream_t s) {
     if (prec == 2) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF64>(items, chunks, n_slots, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF64>(items, chunks, n_slots, cross, idx32, lds, s);
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF64>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF64>(items, chunks, n_slots, threads, cross, idx32, lds, s);
     } else if (prec == 1) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseKahan>(items, chunks, n_slots, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseKahan>(items, chunks, n_slots, cross, idx32, lds, s);
-        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseKahan>(items, chunks, n_slots, cross, idx32, lds, s);
-        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseKahan>(items, chunks, n_slots, cross, idx32, lds, s);
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseKahan>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseKahan>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseKahan>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseKahan>(items, chunks, n_slots, threads, cross, idx32, lds, s);
     } else if (prec == 0) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF32>(items, chunks, n_slots, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF32>(items, chunks, n_slots, cross, idx32, lds, s);
-        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseF32>(items, chunks, n_slots, cross, idx32, lds, s);
-        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseF32>(items, chunks, n_slots, cross, idx32, lds, s);
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF32>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF32>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseF32>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseF32>(items, chunks, n_slots, threads, cross, idx32, lds, s);
     }
     return hipErrorInvalidValue;
 }
@@ -687,12 +687,12 @@ WalkMode walk_mode(const WalkArgs &a, int field) {
     }
 }
 
-hipError_t launch_walk_batch(const WalkArgs *items, const BatchChunk *chunks, uint32_t n_slots, int rng, int field,
-                             int prec, bool cross, bool idx32, size_t lds, hipStream_t s) {
+hipError_t launch_walk_batch(const WalkArgs *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
+                             int rng, int field, int prec, bool cross, bool idx32, size_t lds, hipStream_t s) {
     if (n_slots == 0) return hipSuccess;
-    if (n_slots % 8) return hipErrorInvalidValue;
-    if (rng == 1) return launch_batch_rng<ReplayRng>(items, chunks, n_slots, field, prec, cross, idx32, lds, s);
-    if (rng == 0) return launch_batch_rng<PhiloxRng>(items, chunks, n_slots, field, prec, cross, idx32, lds, s);
+    if (n_slots % 8 || threads == 0 || threads > 256 || threads % 64) return hipErrorInvalidValue;
+    if (rng == 1) return launch_batch_rng<ReplayRng>(items, chunks, n_slots, threads, field, prec, cross, idx32, lds, s);
+    if (rng == 0) return launch_batch_rng<PhiloxRng>(items, chunks, n_slots, threads, field, prec, cross, idx32, lds, s);
     return hipErrorInvalidValue;
 }
 

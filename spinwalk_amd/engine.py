@@ -71,6 +71,13 @@ class SwWalkOpts(C.Structure):
     ]
 
 
+class SwBatchSchedule(C.Structure):
+    _fields_ = [("chunk_spins", C.c_uint32), ("bands_shared", C.c_uint32), ("bands_mixed", C.c_uint32)]
+
+    def as_dict(self):
+        return dict(chunk_spins=self.chunk_spins, bands_shared=self.bands_shared, bands_mixed=self.bands_mixed)
+
+
 class SwBatchItem(C.Structure):
     _fields_ = [
         ("phantom", _P), ("sequence", _P), ("params", SwParams), ("opts", SwWalkOpts),
@@ -105,7 +112,8 @@ EXPORTS = (
     "sw_version", "sw_params_prepare", "sw_timing_scale", "sw_tesla2deg", "sw_init_positions",
     "sw_replay_fill", "sw_phantom_create", "sw_phantom_create_device", "sw_phantom_destroy",
     "sw_phantom_info", "sw_phantom_download", "sw_phantom_generate", "sw_sequence_create", "sw_sequence_destroy", "sw_walk_device", "sw_walk_host",
-    "sw_batch_create", "sw_batch_run", "sw_batch_destroy",
+    "sw_batch_create", "sw_batch_run", "sw_batch_destroy", "sw_batch_get_schedule", "sw_batch_set_schedule",
+    "sw_batch_tune",
     "sw_echo_reduce_device", "sw_philox_draws_device", "sw_probe_gather_device", "sw_stream_sync",
 )
 
@@ -171,6 +179,10 @@ def load():
     L.sw_batch_run.argtypes = [_P, _P, C.POINTER(SwWalkStats), C.POINTER(SwWalkStats)]
     L.sw_batch_destroy.argtypes = [_P]
     L.sw_batch_destroy.restype = None
+    L.sw_batch_get_schedule.argtypes = [_P, C.POINTER(SwBatchSchedule)]
+    L.sw_batch_set_schedule.argtypes = [_P, C.POINTER(SwBatchSchedule)]
+    L.sw_batch_tune.argtypes = [_P, _P, C.c_uint32, C.POINTER(SwBatchSchedule), C.POINTER(C.c_float),
+                                C.POINTER(C.c_float)]
     _lib = L
     return L
 
@@ -585,6 +597,26 @@ class Batch:
         tot = SwWalkStats()
         self.eng._check(self.eng.L.sw_batch_run(self.h, sp, per, C.byref(tot)), "sw_batch_run")
         return dict(total=tot.as_dict(), items=[per[i].as_dict() for i in range(self.n)])
+
+    @property
+    def schedule(self):
+        sc = SwBatchSchedule()
+        self.eng._check(self.eng.L.sw_batch_get_schedule(self.h, C.byref(sc)), "sw_batch_get_schedule")
+        return sc.as_dict()
+
+    def set_schedule(self, chunk_spins, bands_shared, bands_mixed):
+        sc = SwBatchSchedule(int(chunk_spins), int(bands_shared), int(bands_mixed))
+        self.eng._check(self.eng.L.sw_batch_set_schedule(self.h, C.byref(sc)), "sw_batch_set_schedule")
+
+    def tune(self, max_trials=0, stream=None):
+        """The autotuner (sw_batch_tune): times full runs of the batch under
+        candidate schedules and keeps the fastest.  Returns the schedule kept and
+        the run times (ms) of it and of the starting schedule."""
+        sp, _ = _stream_arg(stream)
+        sc, best, first = SwBatchSchedule(), C.c_float(), C.c_float()
+        self.eng._check(self.eng.L.sw_batch_tune(self.h, sp, int(max_trials), C.byref(sc), C.byref(best),
+                                                 C.byref(first)), "sw_batch_tune")
+        return dict(schedule=sc.as_dict(), best_ms=best.value, first_ms=first.value)
 
     def close(self):
         if getattr(self, "h", None):

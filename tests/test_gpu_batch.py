@@ -17,11 +17,12 @@ pytestmark = pytest.mark.gpu
 
 
 def _tile(engine, torch, case, scale_index=0, *, spin_offset=0, n=None, precision=E.SW_PREC_F32,
-          field_format=E.SW_FIELD_F32, rng=E.SW_RNG_PHILOX, formats=(E.SW_FIELD_F32,)):
+          field_format=E.SW_FIELD_F32, rng=E.SW_RNG_PHILOX, formats=(E.SW_FIELD_F32,), ph=None):
     p, xs, gs = case.scale_params(case.scales[scale_index])
     n = case.n_spins - spin_offset if n is None else n
     sl = slice(spin_offset, spin_offset + n)
-    ph = engine.phantom(case.mask, case.field_T, case.fov, formats=formats)
+    if ph is None:
+        ph = engine.phantom(case.mask, case.field_T, case.fov, formats=formats)
     seq = engine.sequence(p, case.tables)
     dt = torch.float64 if precision == E.SW_PREC_F64 else torch.float32
     t = dict(phantom=ph, sequence=seq, params=p, spin_offset=spin_offset, n_local=n, xyz_scale=xs,
@@ -145,3 +146,41 @@ def test_batch_rejects_mixed_modes(engine):
     d = _tile(engine, torch, cs[-1][0])
     with pytest.raises(E.EngineError, match="same field format"):
         engine.batch([a, d])
+
+
+def test_batch_schedules_and_autotuner_keep_results(engine):
+    """The batch autotuner (sw_batch_tune) and explicit schedules change only the
+    order in which chunks are walked: under every schedule, and after tuning,
+    every tile still equals its separate walk bit for bit.  Tiles 0-2 share one
+    phantom (a shared cost class), the others have their own."""
+    import torch
+    cs = _cases()
+    shared = engine.phantom(cs[0][0].mask, cs[0][0].field_T, cs[0][0].fov)
+    tiles = [_tile(engine, torch, cs[0][0], 0, ph=shared), _tile(engine, torch, cs[1][0], 1, ph=shared),
+             _tile(engine, torch, cs[4][0], 1, ph=shared)]
+    tiles += [_tile(engine, torch, c, si, **kw) for c, si, kw in (cs[2], cs[3])]
+    refs = [_separate(engine, torch, t) for t in tiles]
+    b = engine.batch(tiles)
+    assert b.schedule == dict(chunk_spins=256, bands_shared=1 << 20, bands_mixed=16)
+
+    def check_run():
+        _clear(tiles)
+        st = b.run()
+        torch.cuda.synchronize()
+        for t, r, x in zip(tiles, refs, st["items"]):
+            _check(t, r, x)
+
+    for sc in ((128, 64, 4), (64, 1, 1), (256, 16, 64), (128, 1 << 20, 16)):
+        b.set_schedule(*sc)
+        assert b.schedule == dict(chunk_spins=sc[0], bands_shared=sc[1], bands_mixed=sc[2])
+        check_run()
+    res = b.tune(max_trials=8)
+    assert res["schedule"] == b.schedule and res["best_ms"] > 0 and res["first_ms"] > 0
+    assert res["schedule"]["chunk_spins"] in (64, 128, 256)
+    check_run()
+    with pytest.raises(E.EngineError, match="chunk_spins"):
+        b.set_schedule(100, 1, 1)
+    with pytest.raises(E.EngineError, match="chunk_spins"):
+        b.set_schedule(256, 0, 1)
+    check_run()                          # a rejected schedule leaves the batch as it was
+    b.close()
