@@ -88,13 +88,21 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
     log(f"[bench] workload {wl['name']}: generating phantom on the device")
     phs, cases_, pidx, vf = W.build_tiles(eng, wl, formats=(E.SW_FIELD_F32, fmt))
     tiles = []
+    inputs = {}        # one device copy of each distinct start array, shared by the tiles that use it
+
+    def resident(a):
+        key = (a.shape, a.dtype.str, hash(a.tobytes()))
+        if key not in inputs or not np.array_equal(inputs[key][0], a):
+            inputs[key] = (a, torch.from_numpy(a).to(dev))
+        return inputs[key][1]
+
     for case, k in zip(cases_, pidx):
         p, xs, gs = case.scale_params(case.scales[0])
         n = case.n_spins
         z = lambda *sh, dt=torch.float32: torch.zeros(sh, dtype=dt, device=dev)
         tiles.append(dict(phantom=phs[k], sequence=eng.sequence(p, case.tables), params=p, xyz_scale=xs,
                           gradient_scale=gs, spin_offset=0, n_local=n, spatial_order=args.order,
-                          M0=torch.from_numpy(case.M0).to(dev), XYZ0=torch.from_numpy(case.XYZ0).to(dev),
+                          M0=resident(case.M0), XYZ0=resident(case.XYZ0),
                           M1=z(n, case.n_te, 3), T=z(n, case.n_te, dt=torch.uint8), XYZ1=z(n, case.traj_len, 3),
                           echo_sums=z(case.n_te, p.n_substrate, 3, dt=torch.float64), steps=case.steps_per_spin()))
     total_steps = sum(t["n_local"] * t["steps"] for t in tiles)

@@ -247,6 +247,15 @@ SW_API int  sw_batch_create(sw_engine *eng, const sw_batch_item *items, uint32_t
  * entries) and total (nullable) are filled after a stream sync; kernel_ms is the
  * time of the whole batch. */
 SW_API int  sw_batch_run(sw_batch *b, void *stream, sw_walk_stats *per_item, sw_walk_stats *total);
+/* Tiles that walk the same trajectories -- the same phantom, seed, spins, start
+ * buffers (M0, XYZ0 pointers), draw source, tissue tables and event times, with
+ * only RF/excitation parameters, gradient amplitudes, dephasing and outputs
+ * apart, as the phase-cycling and gradient scales of one config are -- are
+ * walked together, up to four per walk, sharing draws, moves and gathers (FP32
+ * and Kahan phases).  Every tile's outputs remain its separate walk's, bit for
+ * bit.  *n_walks: walks of the batch; *n_fused_tiles: tiles inside shared walks.
+ * No counterpart in the reference. */
+SW_API int  sw_batch_info(const sw_batch *b, uint32_t *n_walks, uint32_t *n_fused_tiles);
 SW_API void sw_batch_destroy(sw_batch *b);
 
 /* ---- batch schedule and its autotuner (no counterpart in the reference) -----

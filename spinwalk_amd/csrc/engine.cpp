@@ -78,6 +78,11 @@ struct sw_sequence {
     bool exc_valid = false;
     float key_lin = 0, key_quad = 0; int32_t key_ndummy = -1;
     float RF_FA0 = 0, RF_PH0 = 0;
+    // The tables that shape a walk's trajectory and relaxation, byte for byte:
+    // diffusivity, P_XY, T1/T2 and every event time.  Tiles whose sequences have
+    // equal signatures (and equal draws, geometry and start positions) walk the
+    // same trajectories and can share one fused walk.
+    std::vector<char> traj_sig;
 };
 
 namespace {
@@ -603,6 +608,22 @@ SW_API int sw_sequence_create(sw_engine *e, const sw_params *p, const sw_seq *q,
         r.spd = sin((double)phs * kDeg2Rad);
         r.cpd = cos((double)phs * kDeg2Rad);
     }
+    {
+        auto add = [&](const void *p, size_t n) {
+            const char *c = (const char *)p;
+            sq->traj_sig.insert(sq->traj_sig.end(), c, c + n);
+        };
+        const uint32_t counts[5] = {ns, q->n_rf, q->n_te, q->n_dephasing, q->n_gradient};
+        add(counts, sizeof counts);
+        add(q->diffusivity, ns * sizeof(double));
+        add(q->pXY, (size_t)ns * ns * sizeof(float));
+        add(T1s.data(), ns * sizeof(float)); add(T2s.data(), ns * sizeof(float));
+        add(T1d.data(), ns * sizeof(double)); add(T2d.data(), ns * sizeof(double));
+        add(q->RF_t, q->n_rf * sizeof(int32_t));
+        if (q->n_te) add(q->TE_t, q->n_te * sizeof(int32_t));
+        if (q->n_dephasing) add(q->dephasing_t, q->n_dephasing * sizeof(int32_t));
+        if (q->n_gradient) add(q->gradient_t, q->n_gradient * sizeof(int32_t));
+    }
     hipStream_t s = e->stream;
     SW_HIP(e, dev_upload(&sq->diffusivity, q->diffusivity, ns, s));
     SW_HIP(e, dev_upload(&sq->pXY, q->pXY, (size_t)ns * ns, s));
@@ -795,23 +816,26 @@ struct sw_batch {
     sw_engine *eng = nullptr;
     int device = 0;
     uint32_t n_items = 0;
-    sw::WalkArgs *d_items = nullptr;
+    sw::WalkArgs *d_items = nullptr;             // single walks
+    sw::FusedArgs *d_fused = nullptr;            // fused walks of up to kMaxViews tiles
     sw::BatchChunk *d_chunks = nullptr;
     unsigned long long *d_counters = nullptr;    // [n_items][4]
     sw::Excitation *d_exc = nullptr;             // every tile's excitation table
     uint32_t *d_order = nullptr;                 // every tile's visiting order
     double *d_partial = nullptr;                 // echo-reduction scratch
-    struct Group { bool cross; uint32_t offset, n_slots; };
+    struct Group { bool cross; int views; uint32_t offset, n_slots; };
     std::vector<Group> groups;
     struct Reduce { const void *M1; int f64; const uint8_t *T; uint32_t n, n_te, n_sub; double *out; };
     std::vector<Reduce> reduces;
     int rng = 0, field = 0, prec = 0;
     bool idx32 = false;
     size_t lds = 0;
-    // what the chunk table is built from: per tile, its CROSS_FOV group, spins,
-    // steps per spin and phantom records
-    struct Tile { bool cross; uint32_t n_local; uint64_t cost; const void *rec; };
+    // what the chunk table is built from: per walk unit (a single tile, or a
+    // fused group of tiles), its CROSS_FOV group, spins, steps per spin, phantom
+    // records, views (1: d_items[index]; kMaxViews: d_fused[index])
+    struct Tile { bool cross; uint32_t n_local; uint64_t cost; const void *rec; int views; uint32_t index; };
     std::vector<Tile> tiles;
+    uint32_t n_fused_tiles = 0;                  // tiles walked inside fused units
     sw_batch_schedule sched{};
     size_t chunks_cap = 0;
     hipGraph_t graph = nullptr;
@@ -826,7 +850,8 @@ hipError_t batch_enqueue(const sw_batch *b, hipStream_t s) {
     hipError_t err = sw::launch_zero_u64(b->d_counters, (int)(4 * b->n_items), s);
     if (err != hipSuccess) return err;
     for (const auto &g : b->groups) {
-        err = sw::launch_walk_batch(b->d_items, b->d_chunks + g.offset, g.n_slots, b->sched.chunk_spins, b->rng,
+        const void *items = g.views == 1 ? (const void *)b->d_items : (const void *)b->d_fused;
+        err = sw::launch_walk_batch(items, b->d_chunks + g.offset, g.n_slots, b->sched.chunk_spins, g.views, b->rng,
                                     b->field, b->prec, g.cross, b->idx32, b->lds, s);
         if (err != hipSuccess) return err;
     }
@@ -847,9 +872,10 @@ hipError_t batch_enqueue(const sw_batch *b, hipStream_t s) {
 // 151.2 ms with 16 bands, 155.2 ms with 64.  sw_batch_tune searches around it.
 constexpr sw_batch_schedule kDefaultSchedule = {256u, 1u << 20, 16u};
 
-// The chunk table of a schedule: per CROSS_FOV group, each tile's chunks of
-// sc.chunk_spins spins split into 8 contiguous parts (one per XCD; a part is one
-// spatial region of the tile, since chunks follow the visiting order).  Within
+// The chunk table of a schedule: per CROSS_FOV group and walk kind (single or
+// fused), each walk unit's chunks of sc.chunk_spins spins split into 8
+// contiguous parts (one per XCD; a part is one spatial region of the unit,
+// since chunks follow the visiting order).  Within
 // an XCD's list, tiles of one cost class (a factor of two in steps per spin)
 // take turns band by band through their parts, longer classes first.  The
 // chunks resident on an XCD then start in one band of each tile's part: a
@@ -863,10 +889,11 @@ void build_chunk_table(const sw_batch &b, const sw_batch_schedule &sc, std::vect
     groups.clear();
     *kinds = 0;
     const uint32_t cs = sc.chunk_spins;
-    for (int cross = 0; cross < 2; cross++) {
+    for (int grp = 0; grp < 4; grp++) {
+        const int cross = grp & 1, views = grp < 2 ? 1 : sw::kMaxViews;
         std::vector<uint32_t> idx;
         for (uint32_t i = 0; i < (uint32_t)b.tiles.size(); i++)
-            if (b.tiles[i].cross == (cross != 0) && b.tiles[i].n_local) idx.push_back(i);
+            if (b.tiles[i].cross == (cross != 0) && b.tiles[i].views == views && b.tiles[i].n_local) idx.push_back(i);
         if (idx.empty()) continue;
         auto cost = [&](uint32_t i) { return b.tiles[i].cost; };
         std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return cost(x) > cost(y); });
@@ -897,7 +924,7 @@ void build_chunk_table(const sw_batch &b, const sw_batch_schedule &sc, std::vect
                 const uint32_t c0 = (uint32_t)((uint64_t)nc * k / 8), c1 = (uint32_t)((uint64_t)nc * (k + 1) / 8);
                 for (uint32_t c = c0; c < c1; c++) {
                     const uint32_t band = (uint32_t)((uint64_t)(c - c0) * bands / (c1 - c0));
-                    keys[k].push_back({cls, band, r, i, c});
+                    keys[k].push_back({cls, band, r, b.tiles[i].index, c});
                 }
             }
         }
@@ -914,6 +941,7 @@ void build_chunk_table(const sw_batch &b, const sw_batch_schedule &sc, std::vect
         }
         sw_batch::Group g;
         g.cross = cross != 0;
+        g.views = views;
         g.offset = (uint32_t)table.size();
         g.n_slots = (uint32_t)(8 * per);
         for (auto &l : lists) {
@@ -977,6 +1005,25 @@ unsigned schedule_kinds(const sw_batch &b) {
     return kinds;
 }
 
+// Whether two tiles of a batch walk the same trajectories: the same draws
+// (seed, spins, draw source), geometry, start positions, tissue tables and event
+// times, with outputs apart.  Only RF and excitation parameters, gradient
+// amplitudes, dephasing and the output buffers may differ.
+bool same_trajectory(const sw_batch_item &ix, const sw::WalkArgs &x, const sw_batch_item &iy,
+                     const sw::WalkArgs &y) {
+    auto eq = [](const void *p, const void *q, size_t n) { return std::memcmp(p, q, n) == 0; };
+    return ix.sequence->traj_sig == iy.sequence->traj_sig && ix.opts.spatial_order == iy.opts.spatial_order &&
+           x.rec == y.rec && x.nb_y == y.nb_y && x.nb_z == y.nb_z && eq(&x.t2d, &y.t2d, sizeof x.t2d) &&
+           eq(&x.h2d, &y.h2d, sizeof x.h2d) && eq(&x.t2d_f64, &y.t2d_f64, sizeof x.t2d_f64) && x.nx == y.nx &&
+           x.ny == y.ny && x.nz == y.nz && x.matrix_length == y.matrix_length && eq(x.fov, y.fov, sizeof x.fov) &&
+           eq(x.s2g, y.s2g, sizeof x.s2g) && x.n_timepoints == y.n_timepoints && x.n_dummy == y.n_dummy &&
+           x.timestep_us == y.timestep_us && x.n_spins_total == y.n_spins_total && x.spin_offset == y.spin_offset &&
+           x.n_local == y.n_local && x.seed == y.seed && x.max_iterations == y.max_iterations &&
+           x.cross_fov == y.cross_fov && x.record_traj == y.record_traj && eq(&x.xyz_scale, &y.xyz_scale, 4) &&
+           x.traj_len == y.traj_len && x.M0 == y.M0 && x.XYZ0 == y.XYZ0 && x.rn == y.rn &&
+           x.rn_stride == y.rn_stride && x.ru == y.ru && x.ru_stride == y.ru_stride;
+}
+
 void batch_free(sw_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
@@ -984,8 +1031,8 @@ void batch_free(sw_batch *b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
-    for (void *x : {(void *)b->d_items, (void *)b->d_chunks, (void *)b->d_counters, (void *)b->d_exc,
-                    (void *)b->d_order, (void *)b->d_partial})
+    for (void *x : {(void *)b->d_items, (void *)b->d_fused, (void *)b->d_chunks, (void *)b->d_counters,
+                    (void *)b->d_exc, (void *)b->d_order, (void *)b->d_partial})
         dev_free(x);
     delete b;
 }
@@ -1060,11 +1107,49 @@ SW_API int sw_batch_create(sw_engine *e, const sw_batch_item *items, uint32_t n_
             a.order = dst;
         }
     }
-    SW_HIP(e, dev_upload(&b->d_items, args.data(), args.size(), s));
-
-    for (uint32_t i = 0; i < n_items; i++)
-        b->tiles.push_back({args[i].cross_fov != 0, args[i].n_local,
-                            (uint64_t)(args[i].n_dummy + 1) * args[i].n_timepoints, args[i].rec});
+    // ---- walk units: tiles that walk the same trajectories fuse, up to
+    // kMaxViews at a time (FP32 and Kahan phases; an FP64 view would not fit
+    // the registers)
+    std::vector<sw::WalkArgs> singles;
+    std::vector<sw::FusedArgs> fused;
+    std::vector<bool> taken(n_items, false);
+    for (uint32_t i = 0; i < n_items; i++) {
+        if (taken[i]) continue;
+        taken[i] = true;
+        std::vector<uint32_t> group{i};
+        if (b->prec != SW_PREC_F64)
+            for (uint32_t j = i + 1; j < n_items && group.size() < (size_t)sw::kMaxViews; j++)
+                if (!taken[j] && same_trajectory(items[i], args[i], items[j], args[j])) {
+                    taken[j] = true;
+                    group.push_back(j);
+                }
+        const sw::WalkArgs &a0 = args[i];
+        const uint64_t cost = (uint64_t)(a0.n_dummy + 1) * a0.n_timepoints;
+        if (group.size() == 1) {
+            b->tiles.push_back({a0.cross_fov != 0, a0.n_local, cost, a0.rec, 1, (uint32_t)singles.size()});
+            singles.push_back(a0);
+            continue;
+        }
+        sw::FusedArgs f;
+        std::memset(&f, 0, sizeof f);
+        f.a = a0;
+        f.n_views = (uint32_t)group.size();
+        for (int v = 1; v < sw::kMaxViews; v++) {
+            const bool real = (size_t)v < group.size();
+            const sw::WalkArgs &x = args[real ? group[v] : i];
+            sw::WalkView &w = f.xv[v - 1];
+            w.rf = x.rf; w.exc = x.exc; w.deph_deg = x.deph_deg; w.gx = x.gx; w.gy = x.gy; w.gz = x.gz;
+            w.exc_s = x.exc_s; w.exc_c = x.exc_c; w.exc_sd = x.exc_sd; w.exc_cd = x.exc_cd;
+            w.gradient_scale = x.gradient_scale;
+            w.M1 = x.M1; w.XYZ1 = x.XYZ1; w.T = x.T;
+            w.counters = real ? x.counters : nullptr;      // a repeat of view 0 counts nothing
+        }
+        b->tiles.push_back({a0.cross_fov != 0, a0.n_local, cost, a0.rec, sw::kMaxViews, (uint32_t)fused.size()});
+        fused.push_back(f);
+        b->n_fused_tiles += (uint32_t)group.size();
+    }
+    SW_HIP(e, dev_upload(&b->d_items, singles.data(), singles.size(), s));
+    SW_HIP(e, dev_upload(&b->d_fused, fused.data(), fused.size(), s));
     SW_HIP(e, hipStreamSynchronize(s));        // host tables go out of scope
     SW_HIP(e, hipEventCreate(&b->ev0));
     SW_HIP(e, hipEventCreate(&b->ev1));
@@ -1108,6 +1193,13 @@ SW_API int sw_batch_run(sw_batch *b, void *stream, sw_walk_stats *per_item, sw_w
 }
 
 SW_API void sw_batch_destroy(sw_batch *b) { batch_free(b); }
+
+SW_API int sw_batch_info(const sw_batch *b, uint32_t *n_walks, uint32_t *n_fused_tiles) {
+    if (!b) return SW_E_INVALID;
+    if (n_walks) *n_walks = (uint32_t)b->tiles.size();
+    if (n_fused_tiles) *n_fused_tiles = b->n_fused_tiles;
+    return SW_OK;
+}
 
 SW_API int sw_batch_get_schedule(const sw_batch *b, sw_batch_schedule *out) {
     if (!b || !out) return SW_E_INVALID;

@@ -19,6 +19,7 @@
 #include <math.h>
 #include <type_traits>
 
+#include "fmod360.hpp"
 #include "philox.hpp"
 #include "walk_args.hpp"
 #include "walk_launch.hpp"
@@ -111,8 +112,9 @@ struct PhaseKahan {        // FP32 with Kahan compensation
     __device__ void add_grad(double g) { add((float)g); }
     __device__ float value() const { return v - c; }
     // The rotation angle reduced modulo 360 before the compensation is applied
-    // (fmodf is exact), so a phase of thousands of degrees keeps its low bits.
-    __device__ float angle() const { return fmodf(v, 360.f) - c; }
+    // (an exact remainder, fmod360), so a phase of thousands of degrees keeps
+    // its low bits.
+    __device__ float angle() const { return fmod360(v) - c; }
     __device__ void reset() { v = 0.f; c = 0.f; }
 };
 struct PhaseF64 {
@@ -354,6 +356,19 @@ __device__ __forceinline__ uint32_t next_event(uint32_t a, uint32_t b, uint32_t 
     return min(min(a, b), min(c, d));
 }
 
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    // 64-lane butterfly; carries folded after each level
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t olo = __shfl_xor(lo, o), ohi = __shfl_xor(hi, o);
+        const uint32_t nlo = lo + olo;
+        hi = hi + ohi + (nlo < lo ? 1u : 0u);
+        lo = nlo;
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
 template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
 __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32_t &n_rej) {
     using R = typename Phase::R;
@@ -516,20 +531,7 @@ done:
     return st;
 }
 
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-    // 64-lane butterfly; carries folded after each level
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t olo = __shfl_xor(lo, o), ohi = __shfl_xor(hi, o);
-        const uint32_t nlo = lo + olo;
-        hi = hi + ohi + (nlo < lo ? 1u : 0u);
-        lo = nlo;
-    }
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// One block of 256 lanes walks the spins at positions [t0, t0 + 256) of the
+// One workgroup walks the spins at positions [t0, t0 + blockDim.x) of the
 // launch's visiting order (fewer at the end).
 template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
 __device__ __forceinline__ void walk_block(const WalkArgs &a, uint32_t t0) {
@@ -557,6 +559,255 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, uint32_t t0) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// The fused walk: one trajectory, NV views (FusedArgs).  walk_spin_views is
+// walk_spin with the magnetisation, phase, event tables and outputs held per
+// view; the single walk keeps its own function so that its code, which sits at
+// the register limit, is not perturbed.  Batch tests hold every view to the
+// separate walk of its tile, bit for bit.
+// ---------------------------------------------------------------------------
+// View v of a fused walk: view 0 is the WalkArgs fields, views 1.. are xv[v - 1].
+#define SW_VIEW_FIELD(type, name)                                                          \
+    __device__ __forceinline__ type view_##name(const WalkArgs &a, const WalkView *xv, int v) { \
+        return v == 0 ? a.name : xv[v - 1].name;                                           \
+    }
+SW_VIEW_FIELD(const RfPulse *, rf)
+SW_VIEW_FIELD(const Excitation *, exc)
+SW_VIEW_FIELD(const float *, deph_deg)
+SW_VIEW_FIELD(const float *, gx)
+SW_VIEW_FIELD(const float *, gy)
+SW_VIEW_FIELD(const float *, gz)
+SW_VIEW_FIELD(float, exc_s)
+SW_VIEW_FIELD(float, exc_c)
+SW_VIEW_FIELD(double, exc_sd)
+SW_VIEW_FIELD(double, exc_cd)
+SW_VIEW_FIELD(float, gradient_scale)
+SW_VIEW_FIELD(void *, M1)
+SW_VIEW_FIELD(float *, XYZ1)
+SW_VIEW_FIELD(uint8_t *, T)
+SW_VIEW_FIELD(unsigned long long *, counters)
+#undef SW_VIEW_FIELD
+
+// One spin.  NV views share the trajectory (draws, moves, lookups, gathers,
+// labels and their tissue parameters); each view keeps its own magnetisation
+// and phase, formed by the operations of its own walk (walk_spin above).
+template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32, int NV>
+__device__ int walk_spin_views(const WalkArgs &a, const WalkView *xv, uint32_t li, uint64_t &n_acc,
+                               uint32_t &n_rej) {
+    using R = typename Phase::R;
+    const uint32_t spin = a.spin_offset + li;
+    const uint64_t xoff = 3 * (uint64_t)li * a.traj_len;          // this spin's XYZ1 slots
+    float *const xyz1_0 = a.XYZ1 + xoff;                           // view 0's, kept in registers
+    auto xyz1_of = [&](int v) -> float * { return v == 0 ? xyz1_0 : view_XYZ1(a, xv, v) + xoff; };
+    double xo[3], xn[3];
+    R m[NV][3];
+    for (int i = 0; i < 3; i++) {                                  // kernels.cu:95-99
+        const float x0 = a.XYZ0[3 * (uint64_t)li + i] * a.xyz_scale;
+        xo[i] = xn[i] = (double)x0;
+        for (int v = 0; v < NV; v++) m[v][i] = (R)a.M0[3 * (uint64_t)li + i];
+        if (a.record_traj)
+            for (int v = 0; v < NV; v++) xyz1_of(v)[i] = x0;
+    }
+    Vox<IDX32> vx = locate<Rec, IDX32>(a, xo);                     // kernels.cu:102
+    if (!vx.ok) {                                                  // the reference reads out of bounds here
+        if (!a.record_traj)
+            for (int v = 0; v < NV; v++)
+                for (int i = 0; i < 3; i++) xyz1_of(v)[i] = (float)xo[i];
+        return ST_LOST;
+    }
+    uint8_t ts_old = Rec::label(load_rec<typename Rec::T, IDX32>(a.rec, vx.rec));   // :103
+    double diff = a.diffusivity[ts_old];                           // :104
+    // The reference re-reads T1/T2/diffusivity of ts_old at every voxel change
+    // (:167-169); they only change with ts_old, so they are read when it changes.
+    // Before the first accepted step T1 = T2 = 0 (:91); no event can use them
+    // before that step unless the TR has no time points at all.
+    R field = 0, T1 = 0, T2 = 0;
+    if (a.n_timepoints > 0) {
+        if (sizeof(R) == 8) { T1 = (R)a.T1d[ts_old]; T2 = (R)a.T2d[ts_old]; }
+        else                { T1 = (R)a.T1s[ts_old]; T2 = (R)a.T2s[ts_old]; }
+    }
+    uint32_t itr = 0;
+    Rng rng(a, li, spin);
+    rng.peek();
+    Phase ph[NV];
+    uint64_t slot = 0;
+    // Every early end (lost, stuck or out of draws) leaves through `done`, so the
+    // step loop has one exit edge besides its natural end.
+    int st = ST_OK;
+    uint32_t d = 0, tp = 0;
+
+    for (d = 0; d <= a.n_dummy; ++d) {                              // :107
+        const bool last = (d == a.n_dummy);
+        for (int v = 0; v < NV; v++) {                             // :110-120 excitation
+            const Excitation e = view_exc(a, xv, v)[d];
+            if (sizeof(R) == 8)
+                rot_apply<R>(e.kind_d, (R)view_exc_sd(a, xv, v), (R)view_exc_cd(a, xv, v), (R)e.spd, (R)e.cpd, m[v]);
+            else
+                rot_apply<R>(e.kind, (R)view_exc_s(a, xv, v), (R)view_exc_c(a, xv, v), (R)e.sp, (R)e.cp, m[v]);
+        }
+        RecIdx<IDX32> rec_old = ~(RecIdx<IDX32>)0;                 // :123-126 (ind_old: no voxel)
+        uint32_t old_tp = 0;
+        tp = 0;
+        uint32_t rf_i = 1, te_i = 0, dp_i = 0, gr_i = 0;
+        uint32_t next_rf = (a.n_rf > 1) ? (uint32_t)a.rf[1].t : kNever;
+        uint32_t next_te = (last && a.n_te > 0) ? (uint32_t)a.TE_t[0] : kNever;
+        uint32_t next_dp = (a.n_deph > 0) ? (uint32_t)a.deph_t[0] : kNever;
+        uint32_t next_gr = (a.n_grad > 0) ? (uint32_t)a.grad_t[0] : kNever;
+        uint32_t next_ev = next_event(next_rf, next_te, next_dp, next_gr);
+        for (int v = 0; v < NV; v++) ph[v].reset();
+
+        while (tp < a.n_timepoints) {                              // :128
+            const bool moved = diff != 0.;
+            if (moved) {                                           // :130-137
+                if (!rng.avail) { st = ST_EXHAUSTED; goto done; }
+                move<CROSS>(a, rng.px, rng.py, rng.pz, diff, xo, xn);
+            }
+            vx = locate<Rec, IDX32>(a, xn);                        // :140
+            if (!vx.ok) { st = ST_LOST; goto done; }               // :141-147
+            const bool newvox = vx.rec != rec_old;                 // :150
+            typename Rec::T r{};
+            if (newvox)                                            // one gather: label and field
+                r = load_rec<typename Rec::T, IDX32>(a.rec, vx.rec);
+            if (moved) {                                           // next attempt's draws while the load flies
+                rng.commit();
+                rng.peek();
+            }
+            if (newvox) {
+                const uint8_t ts = Rec::label(r);                  // :152
+                const R fnew = Rec::template field<R>(a, r);       // :166, read speculatively
+                if (ts != ts_old) {                                // :153-163
+                    float u;
+                    if (!rng.uniform(u)) { st = ST_EXHAUSTED; goto done; }
+                    if (u >= a.pXY[ts_old * a.n_sub + ts]) {
+                        ++n_rej;
+                        if (itr++ > a.max_iterations) { st = ST_LOST; goto done; }
+                        continue;
+                    }
+                    ts_old = ts;
+                    if (sizeof(R) == 8) { T1 = (R)a.T1d[ts]; T2 = (R)a.T2d[ts]; }
+                    else                { T1 = (R)a.T1s[ts]; T2 = (R)a.T2s[ts]; }
+                    diff = a.diffusivity[ts];
+                }
+                rec_old = vx.rec;                                  // :165
+                field = fnew;
+            }
+            for (int v = 0; v < NV; v++) ph[v].add(field);         // :171
+            itr = 0;
+
+            if (tp == next_ev) {
+                if (tp == next_dp) {                               // :175-178
+                    for (int v = 0; v < NV; v++) {
+                        const float dd = view_deph_deg(a, xv, v)[dp_i];
+                        if (sizeof(R) == 8)
+                            ph[v].add((R)((double)spin * dd / (double)a.n_spins_total));
+                        else
+                            ph[v].add((R)((float)spin * dd / (float)a.n_spins_total));
+                    }
+                    ++dp_i;
+                    next_dp = (dp_i < a.n_deph) ? (uint32_t)a.deph_t[dp_i] : kNever;
+                }
+                if (tp == next_gr) {                               // :181-187
+                    for (int v = 0; v < NV; v++) {
+                        const float gs = view_gradient_scale(a, xv, v);
+                        const float Gx = view_gx(a, xv, v)[gr_i] * gs;
+                        const float Gy = view_gy(a, xv, v)[gr_i] * gs;
+                        const float Gz = view_gz(a, xv, v)[gr_i] * gs;
+                        ph[v].add_grad((Gx * xn[0] + Gy * xn[1] + Gz * xn[2]) * 1e-3 * a.timestep_us * 1e-6 *
+                                       kGamma * kRad2Deg);
+                    }
+                    ++gr_i;
+                    next_gr = (gr_i < a.n_grad) ? (uint32_t)a.grad_t[gr_i] : kNever;
+                }
+                if (tp == next_rf) {                               // :190-199
+                    const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
+                    for (int v = 0; v < NV; v++) {
+                        dephase_relax(m[v], ph[v].angle(), T1, T2, te);
+                        const RfPulse p = view_rf(a, xv, v)[rf_i];
+                        if (sizeof(R) == 8) rot_apply<R>(p.kind_d, (R)p.sd, (R)p.cd, (R)p.spd, (R)p.cpd, m[v]);
+                        else                rot_apply<R>(p.kind, (R)p.s, (R)p.c, (R)p.sp, (R)p.cp, m[v]);
+                        ph[v].reset();
+                    }
+                    old_tp = tp;
+                    ++rf_i;
+                    next_rf = (rf_i < a.n_rf) ? (uint32_t)a.rf[rf_i].t : kNever;
+                }
+                if (tp == next_te) {                               // :202-215
+                    const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
+                    const uint64_t o = (uint64_t)li * a.n_te + te_i;
+                    for (int v = 0; v < NV; v++) {
+                        dephase_relax(m[v], ph[v].angle(), T1, T2, te);
+                        R *M1 = (R *)view_M1(a, xv, v);
+                        M1[3 * o + 0] = m[v][0]; M1[3 * o + 1] = m[v][1]; M1[3 * o + 2] = m[v][2];
+                        view_T(a, xv, v)[o] = ts_old;
+                        ph[v].reset();
+                    }
+                    old_tp = tp;
+                    ++te_i;
+                    next_te = (te_i < a.n_te) ? (uint32_t)a.TE_t[te_i] : kNever;
+                }
+                next_ev = next_event(next_rf, next_te, next_dp, next_gr);
+            }
+            if (a.record_traj) {                                   // :218-221
+                for (int v = 0; v < NV; v++) {
+                    float *dst = xyz1_of(v) + 3 * slot;
+                    dst[0] = (float)xn[0]; dst[1] = (float)xn[1]; dst[2] = (float)xn[2];
+                }
+                ++slot;
+            }
+            xo[0] = xn[0]; xo[1] = xn[1]; xo[2] = xn[2];
+            ++tp;
+        }
+        const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);   // :226-231
+        for (int v = 0; v < NV; v++) dephase_relax(m[v], ph[v].angle(), T1, T2, te);
+    }
+    d = a.n_dummy + 1;                                             // every TR done
+    tp = 0;
+done:
+    // Without trajectories XYZ1 holds the last accepted position: the reference
+    // stores every accepted position in place (kernels.cu:222-223), and a spin
+    // that is lost keeps the one before.  Then the count of accepted steps.
+    if (!a.record_traj)
+        for (int v = 0; v < NV; v++)
+            for (int i = 0; i < 3; i++) xyz1_of(v)[i] = (float)xo[i];
+    n_acc += (uint64_t)d * a.n_timepoints + tp;
+    return st;
+}
+
+
+// One workgroup walks the spins at positions [t0, t0 + blockDim.x) of the
+// launch's visiting order (fewer at the end).
+template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32, int NV>
+__device__ __forceinline__ void walk_block_views(const WalkArgs &a, const WalkView *xv, uint32_t n_views,
+                                                 uint32_t t0) {
+    const uint32_t t = t0 + threadIdx.x;
+    if (IDX32) {
+        const uint32_t n_tab = (uint32_t)(a.nx + a.ny + a.nz);
+        for (uint32_t i = threadIdx.x; i < n_tab; i += blockDim.x) axis_tab[i] = axis_entry<Rec>(a, i);
+        __syncthreads();
+    }
+    uint64_t n_acc = 0;
+    uint32_t n_rej = 0, lost = 0, exh = 0;
+    if (t < a.n_local) {
+        const uint32_t li = a.order ? a.order[t] : t;
+        const int st = walk_spin_views<Rng, Rec, Phase, CROSS, IDX32, NV>(a, xv, li, n_acc, n_rej);
+        lost = (st == ST_LOST);
+        exh = (st == ST_EXHAUSTED);
+    }
+    const uint64_t s_acc = wave_sum_u64(n_acc), s_rej = wave_sum_u64(n_rej);
+    const uint64_t s_lost = wave_sum_u64(lost), s_exh = wave_sum_u64(exh);
+    if ((threadIdx.x & 63) == 0) {
+        for (int v = 0; v < NV; v++) {                 // every view walked these steps
+            unsigned long long *c = view_counters(a, xv, v);
+            if (c && (v == 0 || (uint32_t)v < n_views)) {
+                if (s_lost) atomicAdd(&c[0], (unsigned long long)s_lost);
+                if (s_exh)  atomicAdd(&c[1], (unsigned long long)s_exh);
+                atomicAdd(&c[2], (unsigned long long)s_acc);
+                atomicAdd(&c[3], (unsigned long long)(s_acc + s_rej));
+            }
+        }
+    }
+}
+
 // Occupancy target: waves per SIMD the register allocation must allow.
 #ifndef SW_WALK_WAVES_PER_EU
 #define SW_WALK_WAVES_PER_EU 6
@@ -576,14 +827,31 @@ void walk_kernel(const WalkArgs a) {
 // The batched walk: block b takes the chunk at slot (b mod 8) * (grid / 8) + b / 8
 // of the chunk table, i.e. XCD k works through slots [k * grid/8, (k+1) * grid/8)
 // in order.  The host lays each XCD's slots out longest tile first, so the long
-// walks start early and the short ones fill in at the end.
-template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SW_WALK_WAVES_PER_EU)))
-void walk_batch_kernel(const WalkArgs *__restrict__ items, const BatchChunk *__restrict__ chunks) {
+// walks start early and the short ones fill in at the end.  NV > 1: fused items
+// of up to NV views, whose per-view state costs registers, so the occupancy
+// target is one wave lower.
+// Occupancy of the fused kernels: the per-view state (magnetisation and one
+// phase register per view, two with Kahan compensation) leaves room for 5 waves
+// per SIMD with FP32 phases and 4 with Kahan phases without spilling in the
+// step loop (measured: 1000^3 bSSFP Kahan + FP16 1181 ms at 5, 1065 ms at 4;
+// 64-phantom FP32 69.6 ms at 5, 74.1 ms at 4).
+template <class Phase, int NV>
+constexpr int walk_waves_per_eu() {
+    return NV == 1 ? SW_WALK_WAVES_PER_EU : (std::is_same<Phase, PhaseKahan>::value ? 4 : 5);
+}
+
+template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32, int NV>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(walk_waves_per_eu<Phase, NV>())))
+void walk_batch_kernel(const void *__restrict__ items, const BatchChunk *__restrict__ chunks) {
     const uint32_t per = gridDim.x / 8, b = blockIdx.x;
     const BatchChunk c = chunks[(b % 8) * per + b / 8];
     if (c.item == kNoItem) return;                    // padding slot (uniform across the block)
-    walk_block<Rng, Rec, Phase, CROSS, IDX32>(items[c.item], c.first);
+    if constexpr (NV == 1) {
+        walk_block<Rng, Rec, Phase, CROSS, IDX32>(((const WalkArgs *)items)[c.item], c.first);
+    } else {
+        const FusedArgs &f = ((const FusedArgs *)items)[c.item];
+        walk_block_views<Rng, Rec, Phase, CROSS, IDX32, NV>(f.a, f.xv, f.n_views, c.first);
+    }
 }
 
 // 32-bit mode: record byte offsets below 2^32 and the per-axis tables within
@@ -614,36 +882,48 @@ hipError_t launch_fp(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
     return hipGetLastError();
 }
 
-template <class Rng, class F, class P>
-hipError_t launch_batch_fp(const WalkArgs *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
+template <class Rng, class F, class P, int NV>
+hipError_t launch_batch_nv(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
                            bool cross, bool idx32, size_t lds, hipStream_t s) {
     const dim3 grid(n_slots), block(threads);
     if (cross) {
-        if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, true>), grid, block, lds, s, items, chunks);
-        else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, false>), grid, block, 0, s, items, chunks);
+        if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, true, NV>), grid, block, lds, s, items, chunks);
+        else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, false, NV>), grid, block, 0, s, items, chunks);
     } else {
-        if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, false, true>), grid, block, lds, s, items, chunks);
-        else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, false, false>), grid, block, 0, s, items, chunks);
+        if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, false, true, NV>), grid, block, lds, s, items, chunks);
+        else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, false, false, NV>), grid, block, 0, s, items, chunks);
     }
     return hipGetLastError();
 }
 
+// views: 1, or kMaxViews for fused items (FP32 and Kahan phases only).
+template <class Rng, class F, class P>
+hipError_t launch_batch_fp(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
+                           int views, bool cross, bool idx32, size_t lds, hipStream_t s) {
+    if (views == 1) return launch_batch_nv<Rng, F, P, 1>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+    if constexpr (sizeof(typename P::R) == 4) {
+        if (views == kMaxViews)
+            return launch_batch_nv<Rng, F, P, kMaxViews>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+    }
+    return hipErrorInvalidValue;
+}
+
 template <class Rng>
-hipError_t launch_batch_rng(const WalkArgs *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                            int field, int prec, bool cross, bool idx32, size_t lds, hipStream_t s) {
+hipError_t launch_batch_rng(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
+                            int views, int field, int prec, bool cross, bool idx32, size_t lds, hipStream_t s) {
     if (prec == 2) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF64>(items, chunks, n_slots, threads, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF64>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF64>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF64>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
     } else if (prec == 1) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseKahan>(items, chunks, n_slots, threads, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseKahan>(items, chunks, n_slots, threads, cross, idx32, lds, s);
-        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseKahan>(items, chunks, n_slots, threads, cross, idx32, lds, s);
-        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseKahan>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
+        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
+        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
     } else if (prec == 0) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF32>(items, chunks, n_slots, threads, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF32>(items, chunks, n_slots, threads, cross, idx32, lds, s);
-        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseF32>(items, chunks, n_slots, threads, cross, idx32, lds, s);
-        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseF32>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
+        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
+        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
     }
     return hipErrorInvalidValue;
 }
@@ -687,12 +967,15 @@ WalkMode walk_mode(const WalkArgs &a, int field) {
     }
 }
 
-hipError_t launch_walk_batch(const WalkArgs *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                             int rng, int field, int prec, bool cross, bool idx32, size_t lds, hipStream_t s) {
+hipError_t launch_walk_batch(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
+                             int views, int rng, int field, int prec, bool cross, bool idx32, size_t lds,
+                             hipStream_t s) {
     if (n_slots == 0) return hipSuccess;
     if (n_slots % 8 || threads == 0 || threads > 256 || threads % 64) return hipErrorInvalidValue;
-    if (rng == 1) return launch_batch_rng<ReplayRng>(items, chunks, n_slots, threads, field, prec, cross, idx32, lds, s);
-    if (rng == 0) return launch_batch_rng<PhiloxRng>(items, chunks, n_slots, threads, field, prec, cross, idx32, lds, s);
+    if (rng == 1)
+        return launch_batch_rng<ReplayRng>(items, chunks, n_slots, threads, views, field, prec, cross, idx32, lds, s);
+    if (rng == 0)
+        return launch_batch_rng<PhiloxRng>(items, chunks, n_slots, threads, views, field, prec, cross, idx32, lds, s);
     return hipErrorInvalidValue;
 }
 

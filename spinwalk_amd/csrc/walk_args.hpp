@@ -49,6 +49,27 @@ constexpr BrickShape kBrick4 = {2, 2, 1};
 
 SW_HD_INLINE uint64_t brick_count(uint64_t n, int lb) { return (n + (1ull << lb) - 1) >> lb; }
 
+// A fused walk carries up to kMaxViews views: tiles whose spins walk the same
+// trajectories (the same draws, geometry, start positions, tissue tables and
+// event times) and differ only in RF and excitation phases or flip angles,
+// gradient amplitudes or dephasing, and in their output buffers.  View 0 is the
+// WalkArgs fields themselves; views 1.. are FusedArgs::xv.  Each view's
+// magnetisation and phase are formed by exactly the operations of its own walk.
+constexpr int kMaxViews = 4;
+struct WalkView {
+    const RfPulse    *rf;
+    const Excitation *exc;
+    const float      *deph_deg;
+    const float      *gx, *gy, *gz;
+    float             exc_s, exc_c;
+    double            exc_sd, exc_cd;
+    float             gradient_scale;
+    void             *M1;
+    float            *XYZ1;
+    uint8_t          *T;
+    unsigned long long *counters;
+};
+
 struct WalkArgs {
     // ---- phantom -----------------------------------------------------------
     const void     *rec;          // bricked voxel records (Rec8 or Rec4)
@@ -101,8 +122,17 @@ struct WalkArgs {
     unsigned long long *counters;
 };
 
-// A work unit of the batched walk: positions [first, first + 256) of one item's
-// visiting order.  Padding slots carry item == kNoItem.
+// An item of the fused batched walk: view 0 is `a`, views 1..n_views-1 are xv;
+// entries of xv beyond them repeat view 0 with counters = NULL, so the kernel
+// walks kMaxViews views and the extra ones rewrite view 0's values.
+struct FusedArgs {
+    WalkArgs a;
+    uint32_t n_views;
+    WalkView xv[kMaxViews - 1];
+};
+
+// A work unit of the batched walk: positions [first, first + chunk) of one
+// item's visiting order.  Padding slots carry item == kNoItem.
 struct BatchChunk { uint32_t item, first; };
 constexpr uint32_t kNoItem = 0xFFFFFFFFu;
 

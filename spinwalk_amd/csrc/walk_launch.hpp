@@ -19,10 +19,12 @@ WalkMode walk_mode(const WalkArgs &a, int field);
 
 // The batched walk over a chunk table of n_slots entries (a multiple of 8),
 // one workgroup of `threads` lanes (64, 128 or 256) per chunk of as many spins:
-// items[] and chunks[] in device memory; every item shares rng, field, prec,
-// cross and the index mode (idx32 with `lds` bytes covering every item's tables).
-hipError_t launch_walk_batch(const WalkArgs *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                             int rng, int field, int prec, bool cross, bool idx32, size_t lds, hipStream_t s);
+// items[] (WalkArgs, or FusedArgs when views == kMaxViews; prec 0 or 1 only)
+// and chunks[] in device memory; every item shares rng, field, prec, cross and
+// the index mode (idx32 with `lds` bytes covering every item's tables).
+hipError_t launch_walk_batch(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
+                             int views, int rng, int field, int prec, bool cross, bool idx32, size_t lds,
+                             hipStream_t s);
 
 // Production draws of spins [spin0, spin0+n_spins), attempts [0, n_att).
 hipError_t launch_philox_draws(uint64_t seed, uint32_t spin0, uint32_t n_spins, uint32_t n_att,

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _tile(engine, torch, case, scale_index=0, *, spin_offset=0, n=None, precision=E.SW_PREC_F32,
-          field_format=E.SW_FIELD_F32, rng=E.SW_RNG_PHILOX, formats=(E.SW_FIELD_F32,), ph=None):
+          field_format=E.SW_FIELD_F32, rng=E.SW_RNG_PHILOX, formats=(E.SW_FIELD_F32,), ph=None, shared=None):
     p, xs, gs = case.scale_params(case.scales[scale_index])
     n = case.n_spins - spin_offset if n is None else n
     sl = slice(spin_offset, spin_offset + n)
@@ -37,6 +37,8 @@ def _tile(engine, torch, case, scale_index=0, *, spin_offset=0, n=None, precisio
         steps = case.steps_per_spin()
         nn, uu = E.replay_fill(int(p.seed), spin_offset, n, 9 * steps + 96, 3 * steps + 64)
         t["replay"] = (torch.from_numpy(nn).cuda(), torch.from_numpy(uu).cuda())
+    if shared is not None:                     # another tile's start and draw buffers (same contents)
+        t.update(shared)
     return t
 
 
@@ -184,3 +186,45 @@ def test_batch_schedules_and_autotuner_keep_results(engine):
         b.set_schedule(256, 0, 1)
     check_run()                          # a rejected schedule leaves the batch as it was
     b.close()
+
+
+def test_batch_fuses_same_trajectory_tiles(engine):
+    """Gradient scales (PGSE, WHAT_TO_SCALE = 1) and phase-cycling scales (bSSFP,
+    WHAT_TO_SCALE = 2) of one config walk the same trajectories when they share
+    the start and draw buffers: the batch walks them together, four at most per
+    walk (five gradient scales: a walk of four and a single walk; three
+    phase-cycling scales: a walk of three).  Every tile still equals its separate
+    walk bit for bit, in the production, Kahan + FP16 and replay modes."""
+    import torch
+    m1, f1 = cases.cylinder_phantom(n=24, fov_um=48.0, radius_um=5.0, n_cyl=3, seed=41)
+    m3, f3 = cases.blocky_phantom((17, 9, 31), n_labels=2, seed=43)
+    pg, _ = cases.pgse(2, b=800.0, delta_ms=1.0, DELTA_ms=2.0, dt=50, axis=(0.3, 1.0, 0.2), start_ms=0.5,
+                       P_XY=[[1, 0.5], [0.5, 1]], WHAT_TO_SCALE=1, SCALE=[1.0, 0.7, 0.4, 0.1, 0.55])
+    ssfp = cases.bssfp(2, TE=1000, dt=50, dummies=4, lin=90.0, WHAT_TO_SCALE=2, SCALE=[0.0, 1.0, 2.0],
+                       P_XY=[[1, 0.5], [0.5, 1]])
+    fm = (E.SW_FIELD_F32, E.SW_FIELD_F16)
+    for prec, ff, rng in ((E.SW_PREC_F32, E.SW_FIELD_F32, E.SW_RNG_PHILOX),
+                          (E.SW_PREC_F32_KAHAN, E.SW_FIELD_F16, E.SW_RNG_PHILOX),
+                          (E.SW_PREC_F32, E.SW_FIELD_F32, E.SW_RNG_REPLAY)):
+        tiles = []
+        for cfg, m, f, fov, n_sc in ((pg, m1, f1, 48.0, 5), (ssfp, m3, f3, (17.0, 9.0, 31.0), 3)):
+            case = cases.make_case(cfg, m, f, fov, n_spins=1500)
+            ph = engine.phantom(case.mask, case.field_T, case.fov, formats=fm)
+            shared = None
+            for si in range(n_sc):
+                t = _tile(engine, torch, case, si, precision=prec, field_format=ff, rng=rng, ph=ph, shared=shared)
+                if shared is None:
+                    shared = dict(M0=t["M0"], XYZ0=t["XYZ0"])
+                    if "replay" in t:
+                        shared["replay"] = t["replay"]
+                tiles.append(t)
+        refs = [_separate(engine, torch, t) for t in tiles]
+        b = engine.batch(tiles)
+        assert b.info() == dict(walks=3, fused_tiles=7)
+        for _ in range(2):
+            _clear(tiles)
+            st = b.run()
+            torch.cuda.synchronize()
+            for t, r, x in zip(tiles, refs, st["items"]):
+                _check(t, r, x)
+        b.close()
