@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 PRECISIONS = {"f32": 0, "kahan": 1, "f64": 2}
-FIELDS = {"f32": 1, "f16": 2, "bf16": 3}
+FIELDS = {"f32": 1, "f16": 2, "bf16": 3, "q16": 4}
 
 
 def log(*a):

@@ -51,6 +51,8 @@ extern "C" {
 #define SW_FIELD_F32        1   /* tesla per 1 T, float32 (reference layout)         */
 #define SW_FIELD_F16        2   /* float16 x per-phantom scale                       */
 #define SW_FIELD_BF16       3   /* bfloat16 x per-phantom scale                      */
+#define SW_FIELD_Q16        4   /* int16 fixed point x per-phantom scale (max|field| */
+                                /* is 32767 steps); absolute error <= max|field|/65534 */
 
 #define SW_DTYPE_F32        0
 #define SW_DTYPE_F64        1
@@ -145,7 +147,7 @@ SW_API int   sw_replay_fill(uint64_t seed, uint32_t spin_begin, uint32_t n,
 /* ---- phantom (read_phantom + fieldmap upload, monte_carlo.cu:98-123, 247-262) */
 /* mask: uint8 [nx*ny*nz] row-major, z fastest.  field_T: float32 tesla per 1 T of
  * B0, same layout, or NULL.  formats: bitmask of (1<<SW_FIELD_F32) |
- * (1<<SW_FIELD_F16) | (1<<SW_FIELD_BF16) copies to keep in HBM.  Host pointers;
+ * (1<<SW_FIELD_F16) | (1<<SW_FIELD_BF16) | (1<<SW_FIELD_Q16) copies to keep in HBM.  Host pointers;
  * the call is synchronous.  Fails with SW_E_NOMEM when the copies do not fit. */
 SW_API int  sw_phantom_create(sw_engine *eng, const uint8_t *mask, const float *field_T,
                               const uint64_t dims[3], const float fov_m[3], uint32_t formats,

@@ -55,11 +55,12 @@ struct sw_phantom {
     uint64_t n = 0;
     uint8_t *mask = nullptr;
     float *field = nullptr;          // tesla per 1 T
-    float half_scale = 1.f;
+    float half_scale = 1.f;          // float16/bfloat16 records hold field_T / half_scale
+    float q_scale = 1.f;             // int16 records hold rint(field_T / q_scale)
     uint32_t n_labels = 0;
     bool has_field = false;
     // bricked voxel records gathered by the walk (walk_args.hpp)
-    void *rec[4] = {nullptr, nullptr, nullptr, nullptr};   // by SW_FIELD_*: none(label only), f32, f16, bf16
+    void *rec[5] = {};               // by SW_FIELD_*: none (label only), f32, f16, bf16, q16
 };
 
 struct sw_sequence {
@@ -336,6 +337,17 @@ SW_API int sw_replay_fill(uint64_t seed, uint32_t spin_begin, uint32_t n, float 
 // ---------------------------------------------------------------------------
 // phantom
 // ---------------------------------------------------------------------------
+// HBM a phantom takes: mask, float32 field and the bricked records (8 bytes a
+// voxel for float32, 4 for the others and for the label-only records).
+static size_t phantom_bytes(uint64_t n, bool has_field, uint32_t formats) {
+    size_t need = n;
+    if (!has_field) return need + n * 4;
+    need += n * 4;
+    for (int f = SW_FIELD_F32; f <= SW_FIELD_Q16; f++)
+        if (formats & (1u << f)) need += n * (f == SW_FIELD_F32 ? 8 : 4);
+    return need;
+}
+
 static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
     hipStream_t s = e->stream;
     uint32_t *d_stats = nullptr;
@@ -353,9 +365,11 @@ static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
     int ex = 0;
     if (fmax > 0 && std::isfinite(fmax)) { std::frexp(fmax, &ex); ph->half_scale = std::ldexp(1.0f, ex - 15); }
     else ph->half_scale = 1.f;
+    // fixed-point step mapping max|field| to 32767
+    ph->q_scale = (fmax > 0 && std::isfinite(fmax)) ? (float)((double)fmax / 32767.0) : 1.f;
     // bricked records for the walk
     const uint64_t nx = ph->dims[0], ny = ph->dims[1], nz = ph->dims[2];
-    for (int f = 0; f < 4; f++) {
+    for (int f = 0; f <= SW_FIELD_Q16; f++) {
         const bool want = (f == SW_FIELD_NONE) ? !ph->has_field : (ph->has_field && (formats & (1u << f)));
         if (!want) continue;
         const sw::BrickShape sh = f == SW_FIELD_F32 ? sw::kBrick8 : sw::kBrick4;
@@ -364,8 +378,9 @@ static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
                                      sw::brick_count(nz, sh.lbz)) << (sh.lbx + sh.lby + sh.lbz);
         SW_HIP(e, hipMalloc(&ph->rec[f], nrec * recsz));
         SW_HIP(e, hipMemsetAsync(ph->rec[f], 0, nrec * recsz, s));
-        const int kind = f == SW_FIELD_F32 ? 0 : (f == SW_FIELD_F16 ? 1 : (f == SW_FIELD_BF16 ? 2 : 3));
-        SW_HIP(e, sw::launch_records(ph->mask, ph->field, nx, ny, nz, kind, 1.0f / ph->half_scale, ph->rec[f], s));
+        const int kind = f == SW_FIELD_F32 ? 0 : f == SW_FIELD_F16 ? 1 : f == SW_FIELD_BF16 ? 2 : f == SW_FIELD_Q16 ? 4 : 3;
+        SW_HIP(e, sw::launch_records(ph->mask, ph->field, nx, ny, nz, kind,
+                                     1.0f / (f == SW_FIELD_Q16 ? ph->q_scale : ph->half_scale), ph->rec[f], s));
     }
     SW_HIP(e, hipStreamSynchronize(s));
     return SW_OK;
@@ -384,13 +399,7 @@ static int phantom_create_common(sw_engine *e, const uint8_t *mask, const float 
     for (int i = 0; i < 3; i++) { ph->dims[i] = dims[i]; ph->fov[i] = fov[i]; }
     ph->has_field = field != nullptr;
     if (formats == 0) formats = 1u << SW_FIELD_F32;
-    size_t need = ph->n;
-    if (ph->has_field) {
-        need += ph->n * 4;
-        if (formats & (1u << SW_FIELD_F16)) need += ph->n * 2;
-        if (formats & (1u << SW_FIELD_BF16)) need += ph->n * 2;
-    }
-    int rc = check_free_memory(e, need + (64u << 20));
+    int rc = check_free_memory(e, phantom_bytes(ph->n, ph->has_field, formats) + (64u << 20));
     if (rc) return rc;
     hipStream_t s = e->stream;
     SW_HIP(e, hipMalloc((void **)&ph->mask, ph->n));
@@ -522,7 +531,7 @@ SW_API int sw_phantom_generate(sw_engine *e, const uint64_t dims[3], const float
     for (int i = 0; i < 3; i++) { ph->dims[i] = dims[i]; ph->fov[i] = fov_um[i] * 1e-6f; }
     ph->has_field = true;
     if (formats == 0) formats = 1u << SW_FIELD_F32;
-    int rc = check_free_memory(e, ph->n * 9 + (64u << 20));
+    int rc = check_free_memory(e, phantom_bytes(ph->n, true, formats) + (64u << 20));
     if (rc) return rc;
     hipStream_t s = e->stream;
     sw::GenCylinder *dcyl = nullptr;
@@ -718,7 +727,7 @@ int fill_walk_args(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, con
         return fail(e, SW_E_INVALID, "replay mode needs normal and uniform streams");
     if (o->rng != SW_RNG_REPLAY && o->rng != SW_RNG_PHILOX) return fail(e, SW_E_INVALID, "unknown rng");
     int field = ph->has_field ? o->field_format : SW_FIELD_NONE;
-    if (field < 0 || field > 3) return fail(e, SW_E_INVALID, "unknown field format");
+    if (field < 0 || field > SW_FIELD_Q16) return fail(e, SW_E_INVALID, "unknown field format");
     if (!ph->rec[field]) return fail(e, SW_E_INVALID, "the phantom was not created with this field format");
     if (o->precision == SW_PREC_F64 && field > SW_FIELD_F32)
         return fail(e, SW_E_INVALID, "the FP64 path gathers the float32 field");
@@ -732,7 +741,7 @@ int fill_walk_args(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, con
         a.nb_z = (uint32_t)sw::brick_count(ph->dims[2], sh.lbz);
     }
     a.t2d = sw_tesla2deg(p->B0, p->timestep_us);
-    a.h2d = ph->half_scale * a.t2d;
+    a.h2d = (field == SW_FIELD_Q16 ? ph->q_scale : ph->half_scale) * a.t2d;
     a.t2d_f64 = (double)p->B0 * p->timestep_us * 1e-6 * kGamma * kRad2Deg;
     a.nx = (int64_t)ph->dims[0]; a.ny = (int64_t)ph->dims[1]; a.nz = (int64_t)ph->dims[2];
     for (int i = 0; i < 3; i++) a.nd[i] = (double)ph->dims[i];

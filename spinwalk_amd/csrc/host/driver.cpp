@@ -118,8 +118,7 @@ int run_phantom(sw_engine *eng, swhost::Config &c, size_t iph, std::vector<float
 
     const uint64_t dims[3] = {mdims[0], mdims[1], mdims[2]};
     uint32_t formats = 1u << SW_FIELD_F32;
-    if (ro.field_format == SW_FIELD_F16) formats |= 1u << SW_FIELD_F16;
-    if (ro.field_format == SW_FIELD_BF16) formats |= 1u << SW_FIELD_BF16;
+    if (ro.field_format > SW_FIELD_F32 && ro.field_format <= SW_FIELD_Q16) formats |= 1u << ro.field_format;
     sw_phantom *ph = nullptr;
     int rc = sw_phantom_create(eng, (const uint8_t *)mbytes.data(), has_field ? (const float *)fbytes.data() : nullptr,
                                dims, fov, formats, &ph);

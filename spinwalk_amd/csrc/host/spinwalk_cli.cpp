@@ -5,7 +5,7 @@
 // (src/spinwalk.cpp:45-56): -c/--configs (one or more files), -p/--use_cpu,
 // -d/--device, -g/--gpu_info, -v/--version.  -p is refused: this engine has no CPU
 // path.  Extra sim options select the engine's modes: --rng philox|replay,
-// --precision f32|kahan, --field f32|f16|bf16, --no-spatial-order.
+// --precision f32|kahan, --field f32|f16|bf16|q16, --no-spatial-order.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -20,7 +20,7 @@ extern "C" int sw_run_config_ex(sw_engine *eng, const char *path, int32_t rng, i
 static void usage() {
     std::printf(
         "Usage: spinwalk [-v] [-g] sim -c CONFIG... [-d DEVICE] [--rng philox|replay]\n"
-        "                [--precision f32|kahan] [--field f32|f16|bf16] [--no-spatial-order]\n"
+        "                [--precision f32|kahan] [--field f32|f16|bf16|q16] [--no-spatial-order]\n"
         "       spinwalk config -s GRE|SE|bSSFP -p PHANTOM... -e TE_us -t TIMESTEP_us -o OUTPUT\n"
         "       spinwalk dwi -b B... -v X Y Z -d START_ms delta_ms DELTA_ms -c CONFIG\n"
         "  sim                 run the Monte-Carlo simulation of each config in turn\n"
@@ -130,11 +130,25 @@ int main(int argc, char **argv) {
         }
         if (a == "-p" || a == "--use_cpu") { use_cpu = true; continue; }
         if (a == "-d" || a == "--device") { dev = (uint32_t)std::atoi(need("-d")); continue; }
-        if (a == "--rng") { const std::string v = need("--rng"); rng = v == "replay" ? SW_RNG_REPLAY : SW_RNG_PHILOX; continue; }
-        if (a == "--precision") { const std::string v = need("--precision"); prec = v == "kahan" ? SW_PREC_F32_KAHAN : SW_PREC_F32; continue; }
+        if (a == "--rng") {
+            const std::string v = need("--rng");
+            if (v != "philox" && v != "replay") { std::fprintf(stderr, "unknown rng: %s\n", v.c_str()); return 2; }
+            rng = v == "replay" ? SW_RNG_REPLAY : SW_RNG_PHILOX;
+            continue;
+        }
+        if (a == "--precision") {
+            const std::string v = need("--precision");
+            if (v != "f32" && v != "kahan") { std::fprintf(stderr, "unknown precision: %s\n", v.c_str()); return 2; }
+            prec = v == "kahan" ? SW_PREC_F32_KAHAN : SW_PREC_F32;
+            continue;
+        }
         if (a == "--field") {
             const std::string v = need("--field");
-            field = v == "f16" ? SW_FIELD_F16 : (v == "bf16" ? SW_FIELD_BF16 : SW_FIELD_F32);
+            if (v == "f32") field = SW_FIELD_F32;
+            else if (v == "f16") field = SW_FIELD_F16;
+            else if (v == "bf16") field = SW_FIELD_BF16;
+            else if (v == "q16") field = SW_FIELD_Q16;
+            else { std::fprintf(stderr, "unknown field format: %s\n", v.c_str()); return 2; }
             continue;
         }
         if (a == "--no-spatial-order") { order = 0; continue; }

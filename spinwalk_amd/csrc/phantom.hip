@@ -1,12 +1,18 @@
 // phantom.hip -- phantom preparation kernels: label count, field range and the
-// half-precision copies of the fieldmap.
+// 16-bit copies of the fieldmap.
 //
 // The reference keeps the fieldmap in float32 (src/sim/monte_carlo.cu:98-123,
-// 254).  The mixed-precision walk gathers a float16 or bfloat16 copy of
-// field_T / half_scale, where half_scale is a power of two chosen so that the
-// largest |field| lands near 2^14: scaling by a power of two is exact, and it
-// keeps the 1/r^2 and 1/r^3 tails of vessel fields out of the float16
-// subnormal range for as long as possible.
+// 254).  The mixed-precision walk gathers a 16-bit copy:
+// - float16 or bfloat16 of field_T / half_scale, where half_scale is a power of
+//   two chosen so that the largest |field| lands near 2^14: scaling by a power of
+//   two is exact, and it keeps the 1/r^2 and 1/r^3 tails of vessel fields out of
+//   the float16 subnormal range for as long as possible.  The error is relative:
+//   up to 2^-11 of the value for float16;
+// - int16 fixed point rint(field_T / q_scale), q_scale = max|field| / 32767.
+//   The error is absolute, at most max|field| / 65534, so every value above 3 %
+//   of the largest is held more closely than in float16; a field that is one
+//   constant inside every inclusion (cylinders along B0) is held exactly at its
+//   largest value.
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
@@ -50,7 +56,8 @@ __global__ void to_half_kernel(const float *field, uint64_t n, float inv_scale, 
 }
 
 // Row-major mask/field -> bricked records (see walk_args.hpp).  KIND 0: Rec8
-// {float field_T, label}; 1: Rec4 fp16; 2: Rec4 bf16; 3: Rec4 label only.
+// {float field_T, label}; 1: Rec4 fp16; 2: Rec4 bf16; 3: Rec4 label only;
+// 4: Rec4 int16 fixed point.
 template <int LX, int LY, int LZ, int KIND>
 __global__ void records_kernel(const uint8_t *mask, const float *field, uint64_t n, uint32_t ny, uint32_t nz,
                                uint64_t nb_y, uint64_t nb_z, float inv_scale, void *out) {
@@ -68,6 +75,11 @@ __global__ void records_kernel(const uint8_t *mask, const float *field, uint64_t
             if (KIND == 2 && field) {
                 const uint32_t bits = __float_as_uint(field[v] * inv_scale);
                 h = (bits + 0x7FFFu + ((bits >> 16) & 1u)) >> 16;      // round to nearest even
+            }
+            if (KIND == 4 && field) {
+                // nearest step, ties to even; |q| <= 32767 up to the rounding of inv_scale
+                const double q = rint((double)field[v] * (double)inv_scale);
+                h = (uint32_t)(int32_t)fmin(32767.0, fmax(-32767.0, q)) & 0xFFFFu;
             }
             ((uint32_t *)out)[r] = (h & 0xFFFFu) | (lab << 16);
         }
@@ -113,6 +125,9 @@ hipError_t launch_records(const uint8_t *mask, const float *field, uint64_t nx, 
                                (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
         else if (kind == 2)
             hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 2>), g, b, 0, s, mask, field, n,
+                               (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
+        else if (kind == 4)
+            hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 4>), g, b, 0, s, mask, field, n,
                                (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
         else
             hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 3>), g, b, 0, s, mask, field, n,

@@ -161,6 +161,14 @@ struct RecF16 {                          // {fp16 field_T/half_scale, uint8 labe
         return (R)(__half2float(__ushort_as_half((unsigned short)(r & 0xFFFFu))) * a.h2d);
     }
 };
+struct RecQ16 {                          // {int16 rint(field_T/q_scale), uint8 label, pad}
+    static constexpr BrickShape shape = kBrick4;
+    using T = uint32_t;
+    __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
+    template <class R> __device__ static R field(const WalkArgs &a, T r) {
+        return (R)((float)(int32_t)(int16_t)(uint16_t)(r & 0xFFFFu) * a.h2d);
+    }
+};
 struct RecBF16 {                         // {bf16 field_T/half_scale, uint8 label, pad}
     static constexpr BrickShape shape = kBrick4;
     using T = uint32_t;
@@ -919,11 +927,13 @@ hipError_t launch_batch_rng(const void *items, const BatchChunk *chunks, uint32_
         if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
         if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
         if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
+        if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
     } else if (prec == 0) {
         if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
         if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
         if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
         if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
+        if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
     }
     return hipErrorInvalidValue;
 }
@@ -941,11 +951,13 @@ hipError_t launch_rng(const WalkArgs &a, int field, int prec, hipStream_t s) {
         if (field == 1) return launch_fp<Rng, RecF32, PhaseKahan>(a, s, grid, block);
         if (field == 2) return launch_fp<Rng, RecF16, PhaseKahan>(a, s, grid, block);
         if (field == 3) return launch_fp<Rng, RecBF16, PhaseKahan>(a, s, grid, block);
+        if (field == 4) return launch_fp<Rng, RecQ16, PhaseKahan>(a, s, grid, block);
     } else if (prec == 0) {
         if (field == 0) return launch_fp<Rng, RecNone, PhaseF32>(a, s, grid, block);
         if (field == 1) return launch_fp<Rng, RecF32, PhaseF32>(a, s, grid, block);
         if (field == 2) return launch_fp<Rng, RecF16, PhaseF32>(a, s, grid, block);
         if (field == 3) return launch_fp<Rng, RecBF16, PhaseF32>(a, s, grid, block);
+        if (field == 4) return launch_fp<Rng, RecQ16, PhaseF32>(a, s, grid, block);
     }
     return hipErrorInvalidValue;
 }
@@ -963,6 +975,7 @@ WalkMode walk_mode(const WalkArgs &a, int field) {
     case 0: return mode_of<RecNone>(a);
     case 1: return mode_of<RecF32>(a);
     case 2: return mode_of<RecF16>(a);
+    case 4: return mode_of<RecQ16>(a);
     default: return mode_of<RecBF16>(a);
     }
 }

@@ -8,7 +8,7 @@
 
 namespace sw {
 
-// The walk.  rng: 0 Philox, 1 replay.  field: 0 none, 1 f32, 2 f16, 3 bf16.
+// The walk.  rng: 0 Philox, 1 replay.  field: 0 none, 1 f32, 2 f16, 3 bf16, 4 q16.
 // prec: 0 f32, 1 f32+Kahan, 2 f64.
 hipError_t launch_walk(const WalkArgs &a, int rng, int field, int prec, hipStream_t s);
 
@@ -41,7 +41,9 @@ hipError_t launch_gather_probe(const void *buf, uint64_t bytes, uint32_t n_lanes
 hipError_t launch_phantom_stats(const uint8_t *mask, const float *field, uint64_t n,
                                 uint32_t *stats, hipStream_t s);
 // Row-major mask/field -> bricked voxel records: kind 0 Rec8 f32, 1 Rec4 fp16,
-// 2 Rec4 bf16, 3 Rec4 label only.  `out` must be zero-filled for padding.
+// 2 Rec4 bf16, 3 Rec4 label only, 4 Rec4 int16 fixed point.  inv_scale divides
+// the field before the 16-bit kinds round it.  `out` must be zero-filled for
+// padding.
 hipError_t launch_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz, int kind,
                           float inv_scale, void *out, hipStream_t s);
 hipError_t launch_field_to_half(const float *field, uint64_t n, float inv_scale, int bf16,

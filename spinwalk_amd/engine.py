@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libspinwalk_engine.so")
 SW_OK = 0
 SW_RNG_PHILOX, SW_RNG_REPLAY = 0, 1
 SW_PREC_F32, SW_PREC_F32_KAHAN, SW_PREC_F64 = 0, 1, 2
-SW_FIELD_NONE, SW_FIELD_F32, SW_FIELD_F16, SW_FIELD_BF16 = 0, 1, 2, 3
+SW_FIELD_NONE, SW_FIELD_F32, SW_FIELD_F16, SW_FIELD_BF16, SW_FIELD_Q16 = 0, 1, 2, 3, 4
 SW_DTYPE_F32, SW_DTYPE_F64 = 0, 1
 
 _P = C.c_void_p

@@ -123,8 +123,10 @@ def test_batch_replay_mode_and_precisions(engine):
     cs = _cases()
     for prec, ff, rng in ((E.SW_PREC_F32, E.SW_FIELD_F32, E.SW_RNG_REPLAY),
                           (E.SW_PREC_F64, E.SW_FIELD_F32, E.SW_RNG_PHILOX),
-                          (E.SW_PREC_F32_KAHAN, E.SW_FIELD_F16, E.SW_RNG_PHILOX)):
-        fm = (E.SW_FIELD_F32, E.SW_FIELD_F16)
+                          (E.SW_PREC_F32_KAHAN, E.SW_FIELD_F16, E.SW_RNG_PHILOX),
+                          (E.SW_PREC_F32, E.SW_FIELD_Q16, E.SW_RNG_REPLAY),
+                          (E.SW_PREC_F32_KAHAN, E.SW_FIELD_Q16, E.SW_RNG_PHILOX)):
+        fm = (E.SW_FIELD_F32, E.SW_FIELD_F16, E.SW_FIELD_Q16)
         tiles = [_tile(engine, torch, c, si, precision=prec, field_format=ff, rng=rng, formats=fm, **kw)
                  for c, si, kw in cs[:3]]
         refs = [_separate(engine, torch, t) for t in tiles]

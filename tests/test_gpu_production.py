@@ -12,7 +12,9 @@ by what the mathematics fixes:
   - spin echo: |Mxy(TE)| = exp(-TE/T2) for every spin whatever its path;
   - results do not depend on how spins are ordered or split into launches;
   - precision modes: identical trajectories and labels; M within the stated
-    error of the FP64 path.
+    error of the FP64 path; the int16 fixed-point field holds a field that is
+    one constant inside the inclusions at its float32 value, where float16
+    rounds it.
 """
 import math
 
@@ -246,12 +248,13 @@ def test_precision_modes_against_fp64(engine):
     m, f = cases.cylinder_phantom(n=32, fov_um=64.0, radius_um=6.0, n_cyl=4, seed=37)
     cfg = cases.se(2, TE=40000, dt=20, P_XY=[[1, 0.5], [0.5, 1]], T2=[40.0, 20.0], T1=[2000.0, 1600.0])
     case = cases.make_case(cfg, m, f, 64.0, n_spins=20000)
-    fm = (E.SW_FIELD_F32, E.SW_FIELD_F16, E.SW_FIELD_BF16)
+    fm = (E.SW_FIELD_F32, E.SW_FIELD_F16, E.SW_FIELD_BF16, E.SW_FIELD_Q16)
     ref = run(engine, case, precision=E.SW_PREC_F64, formats=fm)
     res = {}
     for name, prec, ff in (("f32", E.SW_PREC_F32, E.SW_FIELD_F32), ("kahan", E.SW_PREC_F32_KAHAN, E.SW_FIELD_F32),
                            ("f16", E.SW_PREC_F32_KAHAN, E.SW_FIELD_F16),
-                           ("bf16", E.SW_PREC_F32_KAHAN, E.SW_FIELD_BF16)):
+                           ("bf16", E.SW_PREC_F32_KAHAN, E.SW_FIELD_BF16),
+                           ("q16", E.SW_PREC_F32_KAHAN, E.SW_FIELD_Q16)):
         r = run(engine, case, precision=prec, field_format=ff, formats=fm)
         np.testing.assert_array_equal(r["T"], ref["T"])
         np.testing.assert_array_equal(r["XYZ1"], ref["XYZ1"])
@@ -263,6 +266,43 @@ def test_precision_modes_against_fp64(engine):
     assert res["kahan"][1] <= res["f32"][1]
     assert res["f16"][0] < 1e-4, res
     assert res["bf16"][0] < 2e-3, res
+    assert res["q16"][0] < 1e-4, res
+
+
+def test_q16_field_holds_a_constant_inside_field(engine):
+    """Cylinders along B0: no field outside them and one constant c inside
+    (2 pi (1-Y) dchi (cos^2 theta - 1/3) at theta = 0).  dchi is set so that c,
+    after the power-of-two scale, falls 7 units of 16 above a float16 value
+    (c / half_scale = 16391, relative rounding error 4.3e-4).  The int16 copy
+    holds c as its largest step, so its walk matches the float32 field: the mean
+    transverse magnetisation, compared as a complex number (a common phase error
+    counts), stays within 1e-5 of the FP64 walk, at least ten times closer than
+    the float16 copy."""
+    _, f0 = cases.cylinder_phantom(n=32, fov_um=64.0, radius_um=8.0, n_cyl=4, seed=42, random_axis=False)
+    c0 = float(np.abs(f0).max())
+    mant, ex = math.frexp(c0)
+    dchi = 0.11e-6 * 0.4 * (16391.0 / 32768.0) / mant
+    m, f = cases.cylinder_phantom(n=32, fov_um=64.0, radius_um=8.0, n_cyl=4, seed=42, random_axis=False, dchi=dchi)
+    vals = np.unique(f)
+    assert len(vals) == 2 and vals[0] == 0.0
+    c = float(vals[1])
+    hs = math.ldexp(1.0, math.frexp(c)[1] - 15)
+    assert abs(float(np.float16(c / hs)) - c / hs) / (c / hs) > 4e-4      # float16 rounds c by 4.3e-4
+    cfg = cases.gre(2, TE=20000, dt=50, P_XY=[[1, 0.5], [0.5, 1]], T2=[40.0, 40.0], T1=[2000.0, 2000.0])
+    case = cases.make_case(cfg, m, f, 64.0, n_spins=20000)
+    fm = (E.SW_FIELD_F32, E.SW_FIELD_F16, E.SW_FIELD_Q16)
+    ref = run(engine, case, precision=E.SW_PREC_F64, formats=fm)
+    z_ref = ref["M1"][:, 0, 0].astype(np.float64).mean() + 1j * ref["M1"][:, 0, 1].astype(np.float64).mean()
+    assert abs(z_ref) > 0.1
+    err = {}
+    for name, ff in (("f16", E.SW_FIELD_F16), ("q16", E.SW_FIELD_Q16)):
+        r = run(engine, case, precision=E.SW_PREC_F32_KAHAN, field_format=ff, formats=fm)
+        np.testing.assert_array_equal(r["T"], ref["T"])
+        np.testing.assert_array_equal(r["XYZ1"], ref["XYZ1"])
+        z = r["M1"][:, 0, 0].astype(np.float64).mean() + 1j * r["M1"][:, 0, 1].astype(np.float64).mean()
+        err[name] = abs(z - z_ref) / abs(z_ref)
+    assert err["q16"] < 1e-5, err
+    assert err["q16"] * 10 < err["f16"], err
 
 
 def test_echo_reduce_matches_numpy(engine):
