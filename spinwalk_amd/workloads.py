@@ -139,7 +139,7 @@ def pgse_tables(b_first, axis, start_ms, delta_ms, DELTA_ms, dt):
 
 
 def c5():
-    """Heterogeneous batch: 64 phantoms of 128^3-384^3 voxels (1 um voxels) of
+    """Heterogeneous batch: 64 phantoms of 128^3-512^3 voxels (1 um voxels) of
     random cylinders, spheres or packed parallel axons (r 0.5-3 um, 30-70 %), each
     walked with one of GRE (TE 20 ms), SE (TE 40 ms) or PGSE along x, y or z
     (delta 10 ms, Delta 30 ms, b = 1000, 0, 2000, 3000 s/mm^2 as gradient scales of
@@ -148,7 +148,7 @@ def c5():
     rng = np.random.default_rng(2024)
     phantoms, tiles = [], []
     for i in range(64):
-        n = int(rng.choice([128, 192, 256, 320, 384]))
+        n = int(rng.choice([128, 192, 256, 320, 384, 448, 512]))
         kind = i % 3
         if kind == 0:
             spec = cylinders(float(rng.uniform(2.0, 6.0)), float(rng.uniform(2.0, 6.0)), seed=100 + i)
