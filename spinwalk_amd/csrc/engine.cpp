@@ -1292,9 +1292,15 @@ SW_API int sw_batch_tune(sw_batch *b, void *stream, uint32_t max_trials, sw_batc
 SW_API int sw_walk_host(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, const sw_params *p,
                         const sw_walk_opts *o, const float *M0, const float *XYZ0, void *M1, float *XYZ1,
                         uint8_t *T, const float *rn, const float *ru, sw_walk_stats *stats) {
-    if (!e || !o || !p) return fail(e, SW_E_INVALID, "null argument");
+    if (!e || !o || !p || !ph || !q) return fail(e, SW_E_INVALID, "null argument");
+    // checked here as well as by sw_walk_device: the buffer sizes below depend on them
+    if (p->n_dummy_scan < 0) return fail(e, SW_E_INVALID, "n_dummy_scan < 0: call sw_params_prepare first");
+    if (o->n_local && (!M0 || !XYZ0 || !XYZ1 || (q->n_te && (!M1 || !T))))
+        return fail(e, SW_E_INVALID, "spin buffers are required");
+    if (o->rng == SW_RNG_REPLAY && o->n_local && (!rn || !ru))
+        return fail(e, SW_E_INVALID, "replay mode needs normal and uniform streams");
     SW_HIP(e, hipSetDevice(e->device));
-    const size_t n = o->n_local, nte = q ? q->n_te : 0;
+    const size_t n = o->n_local, nte = q->n_te;
     const size_t traj = p->enRecordTrajectory ? (size_t)(p->n_dummy_scan + 1) * p->n_timepoints : 1;
     const size_t m1b = (o->precision == SW_PREC_F64 ? 8 : 4) * 3 * n * nte;
     hipStream_t s = e->stream;

@@ -470,7 +470,34 @@ class Engine:
         self._check(self.L.sw_sequence_create(self.h, C.byref(params), C.byref(q), C.byref(h)),
                     "sw_sequence_create")
         del keep
-        return Sequence(self, h)
+        sq = Sequence(self, h)
+        sq.n_te, sq.n_sub = int(q.n_te), int(q.n_substrate)
+        return sq
+
+    @staticmethod
+    def _check_spin_buffers(seq, params, n, precision, M0, XYZ0, M1, XYZ1, T, replay, rng):
+        """Shapes and dtypes of a walk's buffers (numpy arrays or torch tensors):
+        the C entry points take bare pointers, so a buffer smaller than the walk
+        writes would be overrun."""
+        def shape_dtype(a):
+            return tuple(a.shape), str(a.dtype).replace("torch.", "")
+
+        traj = (params.n_dummy_scan + 1) * params.n_timepoints if params.enRecordTrajectory else 1
+        m1 = "float64" if precision == SW_PREC_F64 else "float32"
+        want = [("M0", M0, (n, 3), "float32"), ("XYZ0", XYZ0, (n, 3), "float32"),
+                ("M1", M1, (n, seq.n_te, 3), m1), ("XYZ1", XYZ1, (n, traj, 3), "float32"),
+                ("T", T, (n, seq.n_te), "uint8")]
+        for name, a, shape, dt in want:
+            got = shape_dtype(a)
+            if got != (shape, dt):
+                raise EngineError(f"{name}: expected shape {shape} {dt}, got {got[0]} {got[1]}")
+        if rng == SW_RNG_REPLAY:
+            if replay is None:
+                raise EngineError("replay mode needs normal and uniform streams")
+            for name, a in zip(("replay normals", "replay uniforms"), replay):
+                sh, dt = shape_dtype(a)
+                if len(sh) != 2 or sh[0] != n or dt != "float32":
+                    raise EngineError(f"{name}: expected shape ({n}, stride) float32, got {sh} {dt}")
 
     # ---- walks
     def walk_host(self, ph, seq, params, M0, XYZ0, M1, XYZ1, T, *, spin_offset=0, xyz_scale=1.0,
@@ -486,9 +513,12 @@ class Engine:
             rn = np.ascontiguousarray(replay[0], np.float32)
             ru = np.ascontiguousarray(replay[1], np.float32)
             o.replay_normal_stride, o.replay_uniform_stride = rn.shape[1], ru.shape[1]
+        self._check_spin_buffers(seq, params, o.n_local, precision, M0, XYZ0, M1, XYZ1, T,
+                                 None if replay is None else (rn, ru), rng)
         st = SwWalkStats()
         for arr in (M0, XYZ0, M1, XYZ1, T):
-            assert arr.flags["C_CONTIGUOUS"]
+            if not arr.flags["C_CONTIGUOUS"]:
+                raise EngineError("walk_host needs C-contiguous arrays")
         rc = self.L.sw_walk_host(self.h, ph.h, seq.h, C.byref(params), C.byref(o), _np_ptr(M0), _np_ptr(XYZ0),
                                  _np_ptr(M1), _np_ptr(XYZ1), _np_ptr(T), _np_ptr(rn), _np_ptr(ru), C.byref(st))
         self._check(rc, "sw_walk_host")
@@ -503,6 +533,11 @@ class Engine:
         o.n_local = int(M0.shape[0] if n_local is None else n_local)
         o.xyz_scale, o.gradient_scale = float(xyz_scale), float(gradient_scale)
         o.rng, o.precision, o.field_format, o.spatial_order = rng, precision, field_format, spatial_order
+        if n_local is None:
+            self._check_spin_buffers(seq, params, o.n_local, precision, M0, XYZ0, M1, XYZ1, T, replay, rng)
+        for t in (M0, XYZ0, M1, XYZ1, T) + (tuple(replay) if replay is not None else ()):
+            if not t.is_contiguous():
+                raise EngineError("walk_device needs contiguous tensors")
         if replay is not None:
             o.replay_normals, o.replay_uniforms = replay[0].data_ptr(), replay[1].data_ptr()
             o.replay_normal_stride, o.replay_uniform_stride = replay[0].shape[1], replay[1].shape[1]
@@ -526,6 +561,15 @@ class Engine:
     def echo_reduce(self, d_M1, d_T, n_sub, d_out, stream=None):
         import torch
         n_spins, n_te = d_T.shape[0], d_T.shape[1]
+        if d_T.dtype != torch.uint8 or d_T.dim() != 2:
+            raise EngineError("T: expected a (n_spins, n_te) uint8 tensor")
+        if tuple(d_M1.shape) != (n_spins, n_te, 3) or d_M1.dtype not in (torch.float32, torch.float64):
+            raise EngineError(f"M1: expected shape {(n_spins, n_te, 3)} float32 or float64, got "
+                              f"{tuple(d_M1.shape)} {d_M1.dtype}")
+        if tuple(d_out.shape) != (n_te, n_sub, 3) or d_out.dtype != torch.float64:
+            raise EngineError(f"echo sums: expected shape {(n_te, n_sub, 3)} float64")
+        if not (d_M1.is_contiguous() and d_T.is_contiguous() and d_out.is_contiguous()):
+            raise EngineError("echo_reduce needs contiguous tensors")
         dt = SW_DTYPE_F64 if d_M1.dtype == torch.float64 else SW_DTYPE_F32
         sp, wait = _stream_arg(stream)
         self._check(self.L.sw_echo_reduce_device(self.h, _P(d_M1.data_ptr()), dt, _P(d_T.data_ptr()), n_spins, n_te,
@@ -577,6 +621,9 @@ class Batch:
             o.rng, o.precision = int(it.get("rng", SW_RNG_PHILOX)), int(it.get("precision", SW_PREC_F32))
             o.field_format, o.spatial_order = int(it.get("field_format", SW_FIELD_F32)), int(it.get("spatial_order", 1))
             rp = it.get("replay")
+            if o.n_local == it["M0"].shape[0]:
+                eng._check_spin_buffers(it["sequence"], it["params"], o.n_local, o.precision, it["M0"], it["XYZ0"],
+                                        it["M1"], it["XYZ1"], it["T"], rp, o.rng)
             if rp is not None:
                 o.replay_normals, o.replay_uniforms = rp[0].data_ptr(), rp[1].data_ptr()
                 o.replay_normal_stride, o.replay_uniform_stride = rp[0].shape[1], rp[1].shape[1]

@@ -316,3 +316,38 @@ def test_echo_reduce_matches_numpy(engine):
     got = out.cpu().numpy()
     want = orc.echo_reduce(M1, T, 4)
     np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-9)
+
+
+def test_walk_rejects_mismatched_buffers(engine):
+    """The C entry points take bare pointers; the bindings refuse buffers whose
+    shape or dtype does not match the walk, and the C side refuses what it can
+    check itself (missing buffers, unprepared parameters)."""
+    m, f = cases.cylinder_phantom(n=16, fov_um=32.0, radius_um=4.0, n_cyl=2, seed=5)
+    case = cases.make_case(cases.gre(2, TE=2000, dt=50), m, f, 32.0, n_spins=64)
+    p, xs, gs = case.scale_params(case.scales[0])
+    ph = engine.phantom(case.mask, case.field_T, case.fov)
+    seq = engine.sequence(p, case.tables)
+    n = case.n_spins
+    good = dict(M0=case.M0, XYZ0=case.XYZ0, M1=np.zeros((n, case.n_te, 3), np.float32),
+                XYZ1=np.zeros((n, case.traj_len, 3), np.float32), T=np.zeros((n, case.n_te), np.uint8))
+
+    def call(precision=E.SW_PREC_F32, params=p, **kw):
+        b = dict(good, **kw)
+        return engine.walk_host(ph, seq, params, b["M0"], b["XYZ0"], b["M1"], b["XYZ1"], b["T"], precision=precision)
+
+    assert call()["n_lost"] == 0
+    with pytest.raises(E.EngineError, match="M1"):
+        call(precision=E.SW_PREC_F64)                                   # float32 M1 for the FP64 path
+    with pytest.raises(E.EngineError, match="XYZ1"):
+        call(XYZ1=np.zeros((n - 1, case.traj_len, 3), np.float32))
+    with pytest.raises(E.EngineError, match="T"):
+        call(T=np.zeros((n, case.n_te), np.int32))
+    with pytest.raises(E.EngineError, match="field format"):
+        engine.walk_host(ph, seq, p, good["M0"], good["XYZ0"], good["M1"], good["XYZ1"], good["T"],
+                         field_format=E.SW_FIELD_Q16)                    # the phantom holds no int16 copy
+    q = E.SwParams.from_buffer_copy(p)
+    q.n_dummy_scan = -1
+    with pytest.raises(E.EngineError, match="n_dummy_scan"):
+        call(params=q)
+    seq.close()
+    ph.close()
