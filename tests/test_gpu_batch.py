@@ -230,3 +230,40 @@ def test_batch_fuses_same_trajectory_tiles(engine):
             for t, r, x in zip(tiles, refs, st["items"]):
                 _check(t, r, x)
         b.close()
+
+
+def test_empty_inputs(engine):
+    """Zero spins everywhere an entry point takes spins: a walk of none is a
+    no-op that reports zero steps, echo sums over no spins are zero (and are
+    written), and an empty tile in a batch leaves the other tiles' results
+    bit-identical to their separate walks."""
+    import torch
+    cs = _cases()
+    case = cs[0][0]
+    p, xs, gs = case.scale_params(case.scales[0])
+    ph = engine.phantom(case.mask, case.field_T, case.fov)
+    seq = engine.sequence(p, case.tables)
+    z = lambda *sh, dt=np.float32: np.zeros(sh, dt)
+    st = engine.walk_host(ph, seq, p, z(0, 3), z(0, 3), z(0, case.n_te, 3), z(0, case.traj_len, 3),
+                          z(0, case.n_te, dt=np.uint8))
+    assert st["n_steps"] == 0 and st["n_lost"] == 0 and st["n_attempts"] == 0
+    zt = lambda *sh, dt=torch.float32: torch.zeros(sh, dtype=dt, device="cuda")
+    st = engine.walk_device(ph, seq, p, zt(0, 3), zt(0, 3), zt(0, case.n_te, 3), zt(0, case.traj_len, 3),
+                            zt(0, case.n_te, dt=torch.uint8))
+    assert st["n_steps"] == 0
+    sums = torch.full((case.n_te, p.n_substrate, 3), 7.0, dtype=torch.float64, device="cuda")
+    engine.echo_reduce(zt(0, case.n_te, 3), zt(0, case.n_te, dt=torch.uint8), p.n_substrate, sums)
+    torch.cuda.synchronize()
+    assert (sums.cpu().numpy() == 0).all()
+    seq.close()
+    empty = _tile(engine, torch, case, n=0)
+    full = _tile(engine, torch, case, ph=empty["phantom"])
+    ref = _separate(engine, torch, full)
+    empty["echo_sums"].fill_(7.0)
+    b = engine.batch([empty, full])
+    st = b.run()
+    torch.cuda.synchronize()
+    assert st["items"][0]["n_steps"] == 0 and st["items"][0]["n_attempts"] == 0
+    assert (empty["echo_sums"].cpu().numpy() == 0).all()
+    _check(full, ref, st["items"][1])
+    b.close()
