@@ -55,11 +55,12 @@ def cpu_baseline(case, ph, budget_s, threads):
         done = e
     rate = steps / wall
     walk_only = steps / max(wall - disc, 1e-9)
+    tail = ("which grows with the spin index, so the full run would be slower" if done < n else
+            "over every spin of the workload")
     return dict(value=rate, unit="spin-steps/s", cores=threads, kind="port",
                 sample=f"spins [0, {done}) of {n} (all {case.steps_per_spin()} steps each, "
                        f"{steps:.3g} spin-steps, {wall:.1f} s wall); the rate includes the reference's per-spin "
-                       f"mt19937 discard(seed+spin), which grows with the spin index, so the full run would be "
-                       f"slower; walk-only rate {walk_only:.4g} spin-steps/s",
+                       f"mt19937 discard(seed+spin), {tail}; walk-only rate {walk_only:.4g} spin-steps/s",
                 walk_only_value=walk_only)
 
 
