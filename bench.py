@@ -398,6 +398,9 @@ def main():
                    "spatial_order": args.order, "rng": "philox4x32-10"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     # the measured line bytes of one launch (rocprof FETCH_SIZE + WRITE_SIZE, calibrated)
+                     # over this run's kernel time: what the memory system moved, in GB/s
+                     "traffic_rate": None if traffic is None else traffic / kernel_s / 1e9,
                      "bytes_per_spin_step": bpe, "kernel_ms": float(np.mean(kms))},
     }
     out["signal_error_vs_f64"] = {"max_rel": max(e["rel_err"] for e in errs) if errs else None, "per_echo_label": errs}
