@@ -38,23 +38,6 @@ __global__ void stats_kernel(const uint8_t *mask, const float *field, uint64_t n
     }
 }
 
-__global__ void to_half_kernel(const float *field, uint64_t n, float inv_scale, int bf16, uint16_t *out) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const float v = field[i] * inv_scale;
-        uint16_t h;
-        if (bf16) {
-            // round to nearest even on the upper 16 bits
-            const uint32_t b = __float_as_uint(v);
-            const uint32_t r = b + 0x7FFFu + ((b >> 16) & 1u);
-            h = (uint16_t)(r >> 16);
-        } else {
-            h = __half_as_ushort(__float2half_rn(v));
-        }
-        out[i] = h;
-    }
-}
-
 // Row-major mask/field -> bricked records (see walk_args.hpp).  KIND 0: Rec8
 // {float field_T, label}; 1: Rec4 fp16; 2: Rec4 bf16; 3: Rec4 label only;
 // 4: Rec4 int16 fixed point.
@@ -94,16 +77,6 @@ hipError_t launch_phantom_stats(const uint8_t *mask, const float *field, uint64_
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(stats_kernel, dim3((unsigned)blocks), dim3(256), 0, s, mask, field, n, stats);
-    return hipGetLastError();
-}
-
-hipError_t launch_field_to_half(const float *field, uint64_t n, float inv_scale, int bf16, uint16_t *out,
-                                hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    uint64_t blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(to_half_kernel, dim3((unsigned)blocks), dim3(256), 0, s, field, n, inv_scale, bf16,
-                       out);
     return hipGetLastError();
 }
 

@@ -37,7 +37,7 @@ hipError_t launch_gather_probe(const void *buf, uint64_t bytes, uint32_t n_lanes
                                uint64_t *sink, hipStream_t s);
 
 // Phantom preparation: max label and max |field| (into stats[0], stats[1] as
-// uint32 bit patterns; stats must be zeroed by the caller), then half copies.
+// uint32 bit patterns; stats must be zeroed by the caller).
 hipError_t launch_phantom_stats(const uint8_t *mask, const float *field, uint64_t n,
                                 uint32_t *stats, hipStream_t s);
 // Row-major mask/field -> bricked voxel records: kind 0 Rec8 f32, 1 Rec4 fp16,
@@ -46,9 +46,6 @@ hipError_t launch_phantom_stats(const uint8_t *mask, const float *field, uint64_
 // padding.
 hipError_t launch_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz, int kind,
                           float inv_scale, void *out, hipStream_t s);
-hipError_t launch_field_to_half(const float *field, uint64_t n, float inv_scale, int bf16,
-                                uint16_t *out, hipStream_t s);
-
 // FP64 echo reduction into out[(te*n_sub+label)*3 + {0,1,2}]; partial is scratch
 // of reduce_partial_len(n_spins, n_te, n_sub) doubles.
 uint64_t reduce_partial_len(uint32_t n_spins, uint32_t n_te, uint32_t n_sub);
