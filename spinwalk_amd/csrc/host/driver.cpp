@@ -17,6 +17,7 @@
 #include <filesystem>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../../include/spinwalk_host.h"
@@ -43,6 +44,27 @@ struct DevBuf {
     ~DevBuf() { if (p) (void)hipFree(p); }
     hipError_t alloc(size_t n) { return n ? hipMalloc(&p, n) : hipSuccess; }
 };
+
+// Host copy of a trajectory chunk (hundreds of MB) split over host threads: one
+// thread copies a few GB/s, well below what the PCIe transfers it sits between
+// move.
+void par_memcpy(void *dst, const void *src, size_t bytes) {
+    constexpr size_t kPiece = 32u << 20;
+    const size_t n_pieces = (bytes + kPiece - 1) / kPiece;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t n_threads = std::min<size_t>({n_pieces, (size_t)hw, 16});
+    if (n_threads <= 1) { std::memcpy(dst, src, bytes); return; }
+    std::vector<std::thread> th;
+    th.reserve(n_threads);
+    for (size_t t = 0; t < n_threads; t++)
+        th.emplace_back([=] {
+            for (size_t i = t; i < n_pieces; i += n_threads) {
+                const size_t off = i * kPiece, len = std::min(kPiece, bytes - off);
+                std::memcpy((char *)dst + off, (const char *)src + off, len);
+            }
+        });
+    for (auto &x : th) x.join();
+}
 
 struct RunOpts {
     int32_t rng = SW_RNG_PHILOX, precision = SW_PREC_F32, field_format = SW_FIELD_F32, spatial_order = 1;
@@ -199,8 +221,8 @@ int run_phantom(sw_engine *eng, swhost::Config &c, size_t iph, std::vector<float
     auto land = [&](int b) -> int {                  // chunk in buffer b -> host XYZ1
         if (!pend[b].live) return SW_OK;
         if (hipEventSynchronize(sp.copied[b]) != hipSuccess) return fail(SW_E_HIP, "trajectory download");
-        std::memcpy(XYZ1.data() + x_slice * pend[b].is + 3 * (size_t)pend[b].s0 * traj, pin[b].p,
-                    12 * (size_t)pend[b].cnt * traj);
+        par_memcpy(XYZ1.data() + x_slice * pend[b].is + 3 * (size_t)pend[b].s0 * traj, pin[b].p,
+                   12 * (size_t)pend[b].cnt * traj);
         pend[b].live = false;
         return SW_OK;
     };
@@ -258,7 +280,7 @@ int run_phantom(sw_engine *eng, swhost::Config &c, size_t iph, std::vector<float
                 if (rc) return rc;
                 // stale host contents first (abandoned spins keep them), then walk
                 const size_t bytes = 12 * (size_t)cnt * traj;
-                std::memcpy(pin[b].p, XYZ1.data() + x_slice * is + 3 * (size_t)s0 * traj, bytes);
+                par_memcpy(pin[b].p, XYZ1.data() + x_slice * is + 3 * (size_t)s0 * traj, bytes);
                 float *dx = (float *)(b ? dXb.p : dXa.p);
                 if (hipMemcpyAsync(dx, pin[b].p, bytes, hipMemcpyHostToDevice, sp.walk) != hipSuccess)
                     return fail(SW_E_HIP, "trajectory upload");
