@@ -350,17 +350,21 @@ static size_t phantom_bytes(uint64_t n, bool has_field, uint32_t formats) {
 
 static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
     hipStream_t s = e->stream;
-    uint32_t *d_stats = nullptr;
-    SW_HIP(e, hipMalloc((void **)&d_stats, 2 * sizeof(uint32_t)));
-    SW_HIP(e, hipMemsetAsync(d_stats, 0, 2 * sizeof(uint32_t), s));
-    SW_HIP(e, sw::launch_phantom_stats(ph->mask, ph->field, ph->n, d_stats, s));
     uint32_t st[2] = {0, 0};
-    SW_HIP(e, hipMemcpyAsync(st, d_stats, sizeof st, hipMemcpyDeviceToHost, s));
-    SW_HIP(e, hipStreamSynchronize(s));
-    dev_free(d_stats);
+    {
+        uint32_t *d_stats = nullptr;
+        SW_HIP(e, hipMalloc((void **)&d_stats, 2 * sizeof(uint32_t)));
+        std::unique_ptr<uint32_t, void (*)(uint32_t *)> guard(d_stats, [](uint32_t *p) { dev_free(p); });
+        SW_HIP(e, hipMemsetAsync(d_stats, 0, 2 * sizeof(uint32_t), s));
+        SW_HIP(e, sw::launch_phantom_stats(ph->mask, ph->field, ph->n, d_stats, s));
+        SW_HIP(e, hipMemcpyAsync(st, d_stats, sizeof st, hipMemcpyDeviceToHost, s));
+        SW_HIP(e, hipStreamSynchronize(s));
+    }
     ph->n_labels = st[0] + 1;                                    // monte_carlo.cu:113
     float fmax;
     std::memcpy(&fmax, &st[1], sizeof fmax);
+    if (ph->has_field && (formats & (1u << SW_FIELD_Q16)) && !std::isfinite(fmax))
+        return fail(e, SW_E_INVALID, "the int16 field copy needs a finite fieldmap");
     // power-of-two scale mapping max|field| to about 2^14
     int ex = 0;
     if (fmax > 0 && std::isfinite(fmax)) { std::frexp(fmax, &ex); ph->half_scale = std::ldexp(1.0f, ex - 15); }
@@ -391,7 +395,7 @@ static int phantom_create_common(sw_engine *e, const uint8_t *mask, const float 
     if (!e || !mask || !dims || !fov || !out) return fail(e, SW_E_INVALID, "null argument");
     *out = nullptr;
     SW_HIP(e, hipSetDevice(e->device));
-    std::unique_ptr<sw_phantom> ph(new sw_phantom);
+    std::unique_ptr<sw_phantom, void (*)(sw_phantom *)> ph(new sw_phantom, sw_phantom_destroy);
     ph->eng = e;
     ph->device = e->device;
     ph->n = dims[0] * dims[1] * dims[2];
@@ -410,7 +414,7 @@ static int phantom_create_common(sw_engine *e, const uint8_t *mask, const float 
     }
     SW_HIP(e, hipStreamSynchronize(s));
     rc = phantom_finish(e, ph.get(), formats);
-    if (rc) { sw_phantom_destroy(ph.release()); return rc; }
+    if (rc) return rc;
     *out = ph.release();
     return SW_OK;
 }
@@ -523,7 +527,7 @@ SW_API int sw_phantom_generate(sw_engine *e, const uint64_t dims[3], const float
         std::vector<uint32_t> fillp(astart.begin(), astart.end() - 1);
         for (size_t k = 0; k < axn.size(); k++) asorted[fillp[acid[k]]++] = axn[k];
     }
-    std::unique_ptr<sw_phantom> ph(new sw_phantom);
+    std::unique_ptr<sw_phantom, void (*)(sw_phantom *)> ph(new sw_phantom, sw_phantom_destroy);
     ph->eng = e;
     ph->device = e->device;
     ph->n = dims[0] * dims[1] * dims[2];
@@ -538,6 +542,10 @@ SW_API int sw_phantom_generate(sw_engine *e, const uint64_t dims[3], const float
     sw::GenSphere *dsph = nullptr;
     uint32_t *dstart = nullptr, *dastart = nullptr;
     sw::GenCylinder *daxn = nullptr;
+    struct Tables {                      // the shape tables, freed on every return
+        void **p[5];
+        ~Tables() { for (void **q : p) dev_free(*q); }
+    } tables{{(void **)&dcyl, (void **)&dsph, (void **)&dstart, (void **)&daxn, (void **)&dastart}};
     SW_HIP(e, hipMalloc((void **)&ph->mask, ph->n));
     SW_HIP(e, hipMalloc((void **)&ph->field, ph->n * sizeof(float)));
     SW_HIP(e, dev_upload(&dcyl, cyl.data(), cyl.size(), s));
@@ -564,9 +572,8 @@ SW_API int sw_phantom_generate(sw_engine *e, const uint64_t dims[3], const float
     g.field = ph->field;
     SW_HIP(e, sw::launch_phantom_gen(g, s));
     SW_HIP(e, hipStreamSynchronize(s));
-    dev_free(dcyl); dev_free(dsph); dev_free(dstart); dev_free(daxn); dev_free(dastart);
     rc = phantom_finish(e, ph.get(), formats);
-    if (rc) { sw_phantom_destroy(ph.release()); return rc; }
+    if (rc) return rc;
     *out = ph.release();
     return SW_OK;
 }

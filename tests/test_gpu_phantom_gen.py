@@ -102,3 +102,16 @@ def test_rejects_bad_specs(engine):
         engine.generate_phantom((8, 8, 8), (8.0,) * 3, [W.cylinders(-1.0, 4.0, seed=1)])
     with pytest.raises(E.EngineError, match="radius_min"):
         engine.generate_phantom((8, 8, 8), (8.0,) * 3, [dict(W.spheres(1.0, 4.0, seed=1), radius_min_um=0.5)])
+
+
+def test_int16_field_copy_needs_a_finite_fieldmap(engine):
+    """The int16 copy scales by max|field|, which a NaN or an infinity leaves
+    undefined: the phantom is refused (the float32 copy of the same field is
+    accepted, as the reference accepts any float fieldmap)."""
+    mask = np.zeros((8, 8, 8), np.uint8)
+    field = np.zeros((8, 8, 8), np.float32)
+    field[3, 4, 5] = np.inf
+    with pytest.raises(E.EngineError, match="finite"):
+        engine.phantom(mask, field, [8e-6] * 3, formats=(E.SW_FIELD_F32, E.SW_FIELD_Q16))
+    ph = engine.phantom(mask, field, [8e-6] * 3, formats=(E.SW_FIELD_F32,))
+    ph.close()
