@@ -236,7 +236,7 @@ SW_API int sw_engine_create(int device_id, sw_engine **out) {
         return fail(nullptr, SW_E_NODEVICE, "hipGetDeviceProperties failed");
     if (std::strncmp(pr.gcnArchName, "gfx950", 6) != 0)
         return fail(nullptr, SW_E_NODEVICE, std::string("device is ") + pr.gcnArchName + ", this build targets gfx950");
-    std::unique_ptr<sw_engine> e(new sw_engine);
+    std::unique_ptr<sw_engine, void (*)(sw_engine *)> e(new sw_engine, sw_engine_destroy);
     e->device = device_id;
     if (hipSetDevice(device_id) != hipSuccess) return fail(nullptr, SW_E_HIP, "hipSetDevice failed");
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
@@ -592,7 +592,7 @@ SW_API int sw_sequence_create(sw_engine *e, const sw_params *p, const sw_seq *q,
         (q->n_gradient && (!q->gradient_t || !q->gradientX_mTm || !q->gradientY_mTm || !q->gradientZ_mTm)))
         return fail(e, SW_E_INVALID, "table pointer missing for a non-empty table");
     SW_HIP(e, hipSetDevice(e->device));
-    std::unique_ptr<sw_sequence> sq(new sw_sequence);
+    std::unique_ptr<sw_sequence, void (*)(sw_sequence *)> sq(new sw_sequence, sw_sequence_destroy);
     sq->eng = e;
     sq->device = e->device;
     const uint32_t ns = q->n_substrate;
@@ -657,7 +657,7 @@ SW_API int sw_sequence_create(sw_engine *e, const sw_params *p, const sw_seq *q,
     SW_HIP(e, dev_upload(&sq->gz, q->gradientZ_mTm, q->n_gradient, s));
     SW_HIP(e, hipStreamSynchronize(s));
     int rc = ensure_exc(e, sq.get(), p, s);
-    if (rc) { sw_sequence_destroy(sq.release()); return rc; }
+    if (rc) return rc;
     *out = sq.release();
     return SW_OK;
 }
