@@ -45,6 +45,9 @@ struct sw_engine {
     void *d_sort = nullptr; size_t sort_bytes = 0;
     uint32_t *d_order = nullptr; size_t order_cap = 0;
     double *d_partial = nullptr; size_t partial_cap = 0;
+    // device copies of sw_walk_host's buffers: M0, XYZ0, M1, XYZ1, T, normals, uniforms
+    void *d_host[7] = {};
+    size_t host_cap[7] = {};
 };
 
 struct sw_phantom {
@@ -253,6 +256,7 @@ SW_API void sw_engine_destroy(sw_engine *e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (void *p : e->d_host) dev_free(p);
     dev_free(e->d_counters);
     dev_free(e->d_sort);
     dev_free(e->d_order);
@@ -1309,51 +1313,52 @@ SW_API int sw_walk_host(sw_engine *e, const sw_phantom *ph, const sw_sequence *q
     SW_HIP(e, hipSetDevice(e->device));
     const size_t n = o->n_local, nte = q->n_te;
     const size_t traj = p->enRecordTrajectory ? (size_t)(p->n_dummy_scan + 1) * p->n_timepoints : 1;
-    const size_t m1b = (o->precision == SW_PREC_F64 ? 8 : 4) * 3 * n * nte;
+    const bool replay = o->rng == SW_RNG_REPLAY;
+    // bytes of each device copy, in the order of sw_engine::d_host
+    const size_t bytes[7] = {12 * n, 12 * n, (o->precision == SW_PREC_F64 ? 8 : 4) * 3 * n * nte, 12 * n * traj,
+                             n * nte, replay ? 4 * n * o->replay_normal_stride : 0,
+                             replay ? 4 * n * o->replay_uniform_stride : 0};
     hipStream_t s = e->stream;
-    float *dM0 = nullptr, *dX0 = nullptr, *dX1 = nullptr, *drn = nullptr, *dru = nullptr;
-    void *dM1 = nullptr;
-    uint8_t *dT = nullptr;
+    // The device copies persist between calls and only grow: a call reuses the
+    // previous call's buffers once the stream has finished with them.
+    size_t grow = 0;
+    for (int i = 0; i < 7; i++) if (bytes[i] > e->host_cap[i]) grow += bytes[i];
+    if (grow) {
+        SW_HIP(e, hipStreamSynchronize(s));
+        for (int i = 0; i < 7; i++)
+            if (bytes[i] > e->host_cap[i]) {
+                dev_free(e->d_host[i]);
+                e->d_host[i] = nullptr;
+                e->host_cap[i] = 0;
+            }
+        int rc = check_free_memory(e, grow);
+        if (rc) return rc;
+        for (int i = 0; i < 7; i++)
+            if (bytes[i] > e->host_cap[i]) {
+                SW_HIP(e, hipMalloc(&e->d_host[i], bytes[i]));
+                e->host_cap[i] = bytes[i];
+            }
+    }
+    float *dM0 = (float *)e->d_host[0], *dX0 = (float *)e->d_host[1], *dX1 = (float *)e->d_host[3];
+    void *dM1 = e->d_host[2];
+    uint8_t *dT = (uint8_t *)e->d_host[4];
+    const void *src[7] = {M0, XYZ0, M1, XYZ1, T, rn, ru};
+    for (int i = 0; i < 7; i++)
+        if (bytes[i]) SW_HIP(e, hipMemcpyAsync(e->d_host[i], src[i], bytes[i], hipMemcpyHostToDevice, s));
     sw_walk_opts oo = *o;
+    if (replay) {
+        oo.replay_normals = (const float *)e->d_host[5];
+        oo.replay_uniforms = (const float *)e->d_host[6];
+    }
     sw_walk_stats st;
     std::memset(&st, 0, sizeof st);
-    int rc = SW_OK;
-    size_t need = n * 24 + m1b + n * nte + traj * n * 12;
-    if (o->rng == SW_RNG_REPLAY) need += 4 * n * (o->replay_normal_stride + o->replay_uniform_stride);
-    rc = check_free_memory(e, need);
-    if (rc) return rc;
-    auto cleanup = [&]() {
-        for (void *x : {(void *)dM0, (void *)dX0, (void *)dX1, (void *)drn, (void *)dru, dM1, (void *)dT}) dev_free(x);
-    };
-#define SW_TRY(expr)                                                                              \
-    do {                                                                                          \
-        hipError_t _e = (expr);                                                                   \
-        if (_e != hipSuccess) { cleanup(); return fail(e, SW_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); } \
-    } while (0)
-    SW_TRY(dev_upload(&dM0, M0, 3 * n, s));
-    SW_TRY(dev_upload(&dX0, XYZ0, 3 * n, s));
-    SW_TRY(dev_upload(&dX1, XYZ1, 3 * n * traj, s));
-    if (m1b) {
-        SW_TRY(hipMalloc(&dM1, m1b));
-        SW_TRY(hipMemcpyAsync(dM1, M1, m1b, hipMemcpyHostToDevice, s));
-    }
-    SW_TRY(dev_upload(&dT, T, n * nte, s));
-    if (o->rng == SW_RNG_REPLAY) {
-        SW_TRY(dev_upload(&drn, rn, n * o->replay_normal_stride, s));
-        SW_TRY(dev_upload(&dru, ru, n * o->replay_uniform_stride, s));
-        oo.replay_normals = drn;
-        oo.replay_uniforms = dru;
-        if (!drn || !dru) { cleanup(); return fail(e, SW_E_INVALID, "replay streams are empty"); }
-    }
-    rc = sw_walk_device(e, ph, q, p, &oo, dM0, dX0, dM1, dX1, dT, s, &st);
+    const int rc = sw_walk_device(e, ph, q, p, &oo, dM0, dX0, dM1, dX1, dT, s, &st);
     if (rc == SW_OK || rc == SW_E_REPLAY) {
-        if (m1b) SW_TRY(hipMemcpyAsync(M1, dM1, m1b, hipMemcpyDeviceToHost, s));
-        if (n * nte != 0) SW_TRY(hipMemcpyAsync(T, dT, n * nte, hipMemcpyDeviceToHost, s));
-        if (n) SW_TRY(hipMemcpyAsync(XYZ1, dX1, 3 * n * traj * sizeof(float), hipMemcpyDeviceToHost, s));
-        SW_TRY(hipStreamSynchronize(s));
+        if (bytes[2]) SW_HIP(e, hipMemcpyAsync(M1, dM1, bytes[2], hipMemcpyDeviceToHost, s));
+        if (bytes[4]) SW_HIP(e, hipMemcpyAsync(T, dT, bytes[4], hipMemcpyDeviceToHost, s));
+        if (bytes[3]) SW_HIP(e, hipMemcpyAsync(XYZ1, dX1, bytes[3], hipMemcpyDeviceToHost, s));
     }
-#undef SW_TRY
-    cleanup();
+    SW_HIP(e, hipStreamSynchronize(s));
     if (stats) *stats = st;
     return rc;
 }
