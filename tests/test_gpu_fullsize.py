@@ -20,9 +20,12 @@ FoV sweep (64 tiles of 1e5 spins, 1 001-20 001 steps) and 64 phantoms (175
 tiles): the mixed-precision batch (Kahan phase, int16 fixed-point field) keeps
 every tile's signals within 1e-4 relative of the FP64 batch of the same draws.
 
-1000^3 bSSFP (four phase-cycling tiles of 1e7 spins, 10 100 steps each): the four
-tiles fuse into one walk, and each tile's M, T and positions are bit-identical
-to its own separate walk.
+1000^3 bSSFP (four phase-cycling tiles of 1e7 spins, 10 100 steps each):
+  - spins [0, 1024) of the phase-cycling tile at scale 1, replayed on the full
+    phantom (its 8 GB of FP32 records take the 64-bit index path), through 100
+    dummy scans: bit-exact against the oracle as above;
+  - the four tiles fuse into one walk, and each tile's M, T and positions are
+    bit-identical to its own separate walk.
 """
 import numpy as np
 import pytest
@@ -191,6 +194,34 @@ def test_mixed_precision_batch_within_1e4_of_fp64(engine, workload):
     bm.close()
     br.close()
     _close(phs, mixed)
+
+
+def test_c3_replay_sample_on_the_full_phantom(engine):
+    wl = W.c3()
+    phs, cases_, _, _ = W.build_tiles(engine, dict(wl, tiles=[wl["tiles"][1]]), formats=(E.SW_FIELD_F32,))
+    ph, case = phs[0], cases_[0]
+    n = 1024
+    steps = case.steps_per_spin()
+    assert steps == 101 * 100 and case.n_spins == 10000000 and case.scale_type == 2
+    mask, field = ph.download(field=True)
+    d = orc.from_engine_case(case, 0, mask=mask, field_T=field)
+    del mask, field
+    # impermeable walls: rejected attempts draw again, so room for three attempts a step
+    normals, uniforms = E.replay_fill(case.params.seed, 0, n, 9 * steps + 96, 3 * steps + 64)
+    p, xs, gs = case.scale_params(case.scales[0])
+    assert p.linear_phase_cycling == 90.0
+    seq = engine.sequence(p, case.tables)
+    o = orc.sim(d, spin_begin=0, spin_end=n, n_threads=8)
+    M1 = np.zeros((n, case.n_te, 3), np.float32)
+    T = np.zeros((n, case.n_te), np.uint8)
+    XYZ1 = np.zeros((n, case.traj_len, 3), np.float32)
+    st = engine.walk_host(ph, seq, p, np.ascontiguousarray(case.M0[:n]), np.ascontiguousarray(case.XYZ0[:n]),
+                          M1, XYZ1, T, xyz_scale=xs, gradient_scale=gs, rng=E.SW_RNG_REPLAY,
+                          field_format=E.SW_FIELD_F32, replay=(normals, uniforms), spatial_order=1)
+    _assert_parity(dict(M1=M1, T=T, XYZ1=XYZ1, stats=st), o, sl=slice(0, n))
+    assert st["n_steps"] == n * steps
+    seq.close()
+    ph.close()
 
 
 def test_c3_fused_batch_equals_separate_walks(engine):
