@@ -17,8 +17,13 @@ three substrates, permeability 0.5):
     sums equal numpy's float64 sums of the downloaded M and T.
 
 FoV sweep (64 tiles of 1e5 spins, 1 001-20 001 steps) and 64 phantoms (175
-tiles): the mixed-precision batch (Kahan phase, int16 fixed-point field) keeps
-every tile's signals within 1e-4 relative of the FP64 batch of the same draws.
+tiles):
+  - a sample of spins of single tiles, replayed on the full phantom, bit-exact
+    against the oracle as above: GRE at FoV scale 2.44 and SE at FoV scale 0.41
+    (8 001 steps) of the sweep; PGSE along z at b = 3000 s/mm^2 (a gradient scale)
+    on the 512^3 packed-axon and the 384^3 sphere phantoms;
+  - the mixed-precision batch (Kahan phase, int16 fixed-point field) keeps every
+    tile's signals within 1e-4 relative of the FP64 batch of the same draws.
 
 1000^3 bSSFP (four phase-cycling tiles of 1e7 spins, 10 100 steps each):
   - spins [0, 1024) of the phase-cycling tile at scale 1, replayed on the full
@@ -194,6 +199,37 @@ def test_mixed_precision_batch_within_1e4_of_fp64(engine, workload):
     bm.close()
     br.close()
     _close(phs, mixed)
+
+
+@pytest.mark.parametrize("workload,tile,n", [("c4", 20, 1024), ("c4", 40, 512), ("c5", 47, 1024),
+                                             ("c5", 100, 1024)])
+def test_tile_replay_sample_on_the_full_phantom(engine, workload, tile, n):
+    wl = W.WORKLOADS[workload]()
+    t = dict(wl["tiles"][tile])
+    if "phantoms" in wl:                       # only the phantom this tile walks
+        wl = dict(wl, phantoms=[wl["phantoms"][t["phantom"]]])
+        t["phantom"] = 0
+    phs, cases_, _, _ = W.build_tiles(engine, dict(wl, tiles=[t]), formats=(E.SW_FIELD_F32,))
+    ph, case = phs[0], cases_[0]
+    steps = case.steps_per_spin()
+    mask, field = ph.download(field=True)
+    d = orc.from_engine_case(case, 0, mask=mask, field_T=field)
+    del mask, field
+    normals, uniforms = E.replay_fill(case.params.seed, 0, n, 9 * steps + 96, 3 * steps + 64)
+    p, xs, gs = case.scale_params(case.scales[0])
+    seq = engine.sequence(p, case.tables)
+    o = orc.sim(d, spin_begin=0, spin_end=n, n_threads=8)
+    M1 = np.zeros((n, case.n_te, 3), np.float32)
+    T = np.zeros((n, case.n_te), np.uint8)
+    XYZ1 = np.zeros((n, case.traj_len, 3), np.float32)
+    st = engine.walk_host(ph, seq, p, np.ascontiguousarray(case.M0[:n]), np.ascontiguousarray(case.XYZ0[:n]),
+                          M1, XYZ1, T, xyz_scale=xs, gradient_scale=gs, rng=E.SW_RNG_REPLAY,
+                          field_format=E.SW_FIELD_F32, replay=(normals, uniforms), spatial_order=1)
+    _assert_parity(dict(M1=M1, T=T, XYZ1=XYZ1, stats=st), o, sl=slice(0, n))
+    if st["n_lost"] == 0:
+        assert st["n_steps"] == n * steps
+    seq.close()
+    ph.close()
 
 
 def test_c3_replay_sample_on_the_full_phantom(engine):
