@@ -840,12 +840,15 @@ void walk_kernel(const WalkArgs a) {
 // target is one wave lower.
 // Occupancy of the fused kernels: the per-view state (magnetisation and one
 // phase register per view, two with Kahan compensation) leaves room for 5 waves
-// per SIMD with FP32 phases and 4 with Kahan phases without spilling in the
-// step loop (measured: 1000^3 bSSFP Kahan + FP16 1181 ms at 5, 1065 ms at 4;
-// 64-phantom FP32 69.6 ms at 5, 74.1 ms at 4).
+// per SIMD, FP32 and Kahan phases alike, with no spill in the step loop; the few
+// spills at 5 waves sit in the TR set-up and the trajectory stores.  Measured:
+// 64-phantom FP32 batch 69.6 ms at 5, 74.1 ms at 4; Kahan + int16 field 68.5 ms
+// at 5, 75.0 ms at 4; 1000^3 bSSFP Kahan + int16 field 486.5 ms at 5, 541.6 ms
+// at 4.  (With the exact remainder by 360 written as a loop, the Kahan kernel
+// spilled in the step loop at 5 and ran slower there.)
 template <class Phase, int NV>
 constexpr int walk_waves_per_eu() {
-    return NV == 1 ? SW_WALK_WAVES_PER_EU : (std::is_same<Phase, PhaseKahan>::value ? 4 : 5);
+    return NV == 1 ? SW_WALK_WAVES_PER_EU : 5;
 }
 
 template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32, int NV>
