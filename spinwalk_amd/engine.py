@@ -475,12 +475,16 @@ class Engine:
         return sq
 
     @staticmethod
-    def _check_spin_buffers(seq, params, n, precision, M0, XYZ0, M1, XYZ1, T, replay, rng):
+    def _check_spin_buffers(seq, params, n, precision, M0, XYZ0, M1, XYZ1, T, replay, rng, exact=True):
         """Shapes and dtypes of a walk's buffers (numpy arrays or torch tensors):
         the C entry points take bare pointers, so a buffer smaller than the walk
-        writes would be overrun."""
+        writes would be overrun.  exact=False accepts buffers with more than n
+        rows (a walk of their first n spins)."""
         def shape_dtype(a):
-            return tuple(a.shape), str(a.dtype).replace("torch.", "")
+            sh, dt = tuple(a.shape), str(a.dtype).replace("torch.", "")
+            if not exact and len(sh) >= 1 and sh[0] >= n:
+                sh = (n,) + sh[1:]
+            return sh, dt
 
         traj = (params.n_dummy_scan + 1) * params.n_timepoints if params.enRecordTrajectory else 1
         m1 = "float64" if precision == SW_PREC_F64 else "float32"
@@ -533,8 +537,8 @@ class Engine:
         o.n_local = int(M0.shape[0] if n_local is None else n_local)
         o.xyz_scale, o.gradient_scale = float(xyz_scale), float(gradient_scale)
         o.rng, o.precision, o.field_format, o.spatial_order = rng, precision, field_format, spatial_order
-        if n_local is None:
-            self._check_spin_buffers(seq, params, o.n_local, precision, M0, XYZ0, M1, XYZ1, T, replay, rng)
+        self._check_spin_buffers(seq, params, o.n_local, precision, M0, XYZ0, M1, XYZ1, T, replay, rng,
+                                 exact=n_local is None)
         for t in (M0, XYZ0, M1, XYZ1, T) + (tuple(replay) if replay is not None else ()):
             if not t.is_contiguous():
                 raise EngineError("walk_device needs contiguous tensors")
@@ -621,9 +625,8 @@ class Batch:
             o.rng, o.precision = int(it.get("rng", SW_RNG_PHILOX)), int(it.get("precision", SW_PREC_F32))
             o.field_format, o.spatial_order = int(it.get("field_format", SW_FIELD_F32)), int(it.get("spatial_order", 1))
             rp = it.get("replay")
-            if o.n_local == it["M0"].shape[0]:
-                eng._check_spin_buffers(it["sequence"], it["params"], o.n_local, o.precision, it["M0"], it["XYZ0"],
-                                        it["M1"], it["XYZ1"], it["T"], rp, o.rng)
+            eng._check_spin_buffers(it["sequence"], it["params"], o.n_local, o.precision, it["M0"], it["XYZ0"],
+                                    it["M1"], it["XYZ1"], it["T"], rp, o.rng, exact=False)
             if rp is not None:
                 o.replay_normals, o.replay_uniforms = rp[0].data_ptr(), rp[1].data_ptr()
                 o.replay_normal_stride, o.replay_uniform_stride = rp[0].shape[1], rp[1].shape[1]
