@@ -886,11 +886,13 @@ hipError_t batch_enqueue(const sw_batch *b, hipStream_t s) {
 // interleave chunk by chunk in relative position (2^20 bands: finer than any
 // part), so tiles that walk the same trajectories (phase-cycling or gradient
 // scales of one seed) run the same spins side by side and share every line;
-// tiles of different phantoms take turns in 16 bands.  Measured: the 1000^3
+// tiles of different phantoms take turns in 4 bands.  Measured: the 1000^3
 // bSSFP batch (4 tiles, one phantom) 2.676 s tile after tile, 2.511 s with 64
-// bands, 2.029 s chunk by chunk; the 64-phantom batch 163.5 ms tile after tile,
-// 151.2 ms with 16 bands, 155.2 ms with 64.  sw_batch_tune searches around it.
-constexpr sw_batch_schedule kDefaultSchedule = {256u, 1u << 20, 16u};
+// bands, 2.029 s chunk by chunk; the 64-phantom batch, walked tile by tile,
+// 163.5 ms tile after tile, 151.2 ms with 16 bands, 155.2 ms with 64; with
+// fused walks the tuner picks 4 bands for FP32 (70.4 ms against 72.2 ms at 16)
+// and 1 for Kahan + int16 (66.5 against 68.3).  sw_batch_tune searches around it.
+constexpr sw_batch_schedule kDefaultSchedule = {256u, 1u << 20, 4u};
 
 // The chunk table of a schedule: per CROSS_FOV group and walk kind (single or
 // fused), each walk unit's chunks of sc.chunk_spins spins split into 8

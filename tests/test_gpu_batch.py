@@ -165,7 +165,7 @@ def test_batch_schedules_and_autotuner_keep_results(engine):
     tiles += [_tile(engine, torch, c, si, **kw) for c, si, kw in (cs[2], cs[3])]
     refs = [_separate(engine, torch, t) for t in tiles]
     b = engine.batch(tiles)
-    assert b.schedule == dict(chunk_spins=256, bands_shared=1 << 20, bands_mixed=16)
+    assert b.schedule == dict(chunk_spins=256, bands_shared=1 << 20, bands_mixed=4)
 
     def check_run():
         _clear(tiles)
