@@ -80,9 +80,18 @@ def test_cli_runs_a_config(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     M = H.h5_read(tmp_path / "res" / "gre_p.h5", "M")
     assert M.shape == (1, 500, 1, 3) and np.isfinite(M).all()
-    # -p (the CPU path) is refused
+    # the mixed-precision options: same trajectories and labels, M within 1e-4 of the default run
+    r = subprocess.run([exe, "sim", "-c", str(tmp_path / "g.ini"), "--precision", "kahan", "--field", "q16"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    M2 = H.h5_read(tmp_path / "res" / "gre_p.h5", "M")
+    np.testing.assert_allclose(M2, M, rtol=0, atol=1e-4)
+    # -p (the CPU path) and unknown option values are refused
     r = subprocess.run([exe, "sim", "-c", str(tmp_path / "g.ini"), "-p"], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
+    r = subprocess.run([exe, "sim", "-c", str(tmp_path / "g.ini"), "--field", "f8"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2 and "unknown field format" in r.stderr
 
 
 def test_generated_phantoms_and_config_run_end_to_end(engine, tmp_path):
