@@ -18,7 +18,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "build", "libsw_oracle.so")
+# SW_ORACLE_LIB points the tests at another build of sw_oracle.cpp (a copy with a
+# planted mistake, when checking that the tests catch it).
+LIB_PATH = os.environ.get("SW_ORACLE_LIB") or os.path.join(_HERE, "build", "libsw_oracle.so")
 
 
 class OrcParams(C.Structure):
