@@ -202,7 +202,10 @@ SW_API void sw_sequence_destroy(sw_sequence *seq);
  * [n_local][n_TE]; XYZ1: float [n_local][traj][3] with traj = (n_dummy+1)*
  * n_timepoints when enRecordTrajectory else 1.  Entries of M1/T of a spin that
  * is abandoned before an echo keep their prior contents (the reference's G9).
- * stats (nullable) is filled after an internal stream sync. */
+ * stats (nullable) is filled after an internal stream sync.  The counters,
+ * visiting order and excitation tables are engine-owned: a call on another
+ * stream than the previous call's waits on the device for that call's use of
+ * them (so walks of one engine on several streams run one after the other). */
 SW_API int  sw_walk_device(sw_engine *eng, const sw_phantom *ph, const sw_sequence *seq,
                            const sw_params *p, const sw_walk_opts *o,
                            const float *d_M0, const float *d_XYZ0,
@@ -258,6 +261,12 @@ SW_API int  sw_batch_run(sw_batch *b, void *stream, sw_walk_stats *per_item, sw_
  * bit.  *n_walks: walks of the batch; *n_fused_tiles: tiles inside shared walks.
  * No counterpart in the reference. */
 SW_API int  sw_batch_info(const sw_batch *b, uint32_t *n_walks, uint32_t *n_fused_tiles);
+/* Whether sw_batch_run launches the batch's captured HIP graph (*uses_graph = 1)
+ * and the nodes of that graph (nullable outputs).  Capture failures are errors
+ * of sw_batch_create / sw_batch_set_schedule / sw_batch_tune, so a batch that
+ * was created runs from its graph.  Runs of one batch on different streams are
+ * ordered one after the other.  No counterpart in the reference. */
+SW_API int  sw_batch_graph_info(const sw_batch *b, int32_t *uses_graph, uint32_t *n_nodes);
 SW_API void sw_batch_destroy(sw_batch *b);
 
 /* ---- batch schedule and its autotuner (no counterpart in the reference) -----

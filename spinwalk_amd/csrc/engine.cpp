@@ -48,6 +48,12 @@ struct sw_engine {
     // device copies of sw_walk_host's buffers: M0, XYZ0, M1, XYZ1, T, normals, uniforms
     void *d_host[7] = {};
     size_t host_cap[7] = {};
+    // Calls that use the engine-owned scratch above (counters, visiting order,
+    // sort and reduction buffers, excitation tables) may come on different
+    // streams.  Each records scratch_ev on its stream when it has enqueued its
+    // last use; the next call on another stream waits for it first.
+    hipEvent_t scratch_ev = nullptr;
+    hipStream_t scratch_stream = nullptr;     // stream of the last recorded use
 };
 
 struct sw_phantom {
@@ -196,6 +202,19 @@ int check_free_memory(sw_engine *e, size_t bytes) {   // device_helper.cu:78-101
     return SW_OK;
 }
 
+// Orders a call's use of the engine-owned scratch after the previous call's,
+// when that call ran on another stream.
+int scratch_acquire(sw_engine *e, hipStream_t s) {
+    if (e->scratch_stream && e->scratch_stream != s) SW_HIP(e, hipStreamWaitEvent(s, e->scratch_ev, 0));
+    return SW_OK;
+}
+// Marks the point on `s` after which the scratch is free for the next call.
+int scratch_release(sw_engine *e, hipStream_t s) {
+    SW_HIP(e, hipEventRecord(e->scratch_ev, s));
+    e->scratch_stream = s;
+    return SW_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -246,7 +265,8 @@ SW_API int sw_engine_create(int device_id, sw_engine **out) {
         return fail(nullptr, SW_E_HIP, "hipStreamCreate failed");
     if (hipMalloc((void **)&e->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess)
         return fail(nullptr, SW_E_HIP, "hipMalloc counters failed");
-    if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
+    if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&e->scratch_ev, hipEventDisableTiming) != hipSuccess)
         return fail(nullptr, SW_E_HIP, "hipEventCreate failed");
     *out = e.release();
     return SW_OK;
@@ -263,6 +283,10 @@ SW_API void sw_engine_destroy(sw_engine *e) {
     dev_free(e->d_partial);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->scratch_ev) {
+        (void)hipEventSynchronize(e->scratch_ev);
+        (void)hipEventDestroy(e->scratch_ev);
+    }
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -802,6 +826,8 @@ SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence 
     int field = 0;
     int rc = fill_walk_args(e, ph, q, p, o, d_M0, d_XYZ0, d_M1, d_XYZ1, d_T, a, &field);
     if (rc) return rc;
+    // the counters, the visiting order and the excitation table are engine-owned
+    if ((rc = scratch_acquire(e, s))) return rc;
     rc = ensure_exc(e, q, p, s);
     if (rc) return rc;
     a.exc = q->exc;
@@ -815,10 +841,13 @@ SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence 
     SW_HIP(e, sw::launch_zero_u64(e->d_counters, 4, s));
     if (stats) SW_HIP(e, hipEventRecord(e->ev0, s));
     SW_HIP(e, sw::launch_walk(a, o->rng == SW_RNG_REPLAY ? 1 : 0, field, o->precision, s));
+    unsigned long long c[4];
     if (stats) {
         SW_HIP(e, hipEventRecord(e->ev1, s));
-        unsigned long long c[4];
         SW_HIP(e, hipMemcpyAsync(c, e->d_counters, sizeof c, hipMemcpyDeviceToHost, s));
+    }
+    if ((rc = scratch_release(e, s))) return rc;
+    if (stats) {
         SW_HIP(e, hipStreamSynchronize(s));
         float ms = 0;
         SW_HIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
@@ -861,6 +890,11 @@ struct sw_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // The chunk table, counters and reduction scratch belong to the batch: a run
+    // on one stream waits for the previous run on another (run_ev), and a new
+    // schedule waits for the last run before it rewrites the chunk table.
+    hipEvent_t run_ev = nullptr;
+    hipStream_t run_stream = nullptr;
 };
 
 namespace {
@@ -987,7 +1021,9 @@ int apply_schedule(sw_engine *e, sw_batch *b, const sw_batch_schedule &sc, hipSt
     std::vector<sw_batch::Group> groups;
     unsigned kinds = 0;
     build_chunk_table(*b, sc, table, groups, &kinds);
-    SW_HIP(e, hipStreamSynchronize(s));          // the previous table may be in use
+    // the previous table may be in use by a run on any stream
+    if (b->run_stream) SW_HIP(e, hipEventSynchronize(b->run_ev));
+    SW_HIP(e, hipStreamSynchronize(s));
     if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
     if (table.size() > b->chunks_cap) {
@@ -1004,17 +1040,25 @@ int apply_schedule(sw_engine *e, sw_batch *b, const sw_batch_schedule &sc, hipSt
     b->groups = groups;
     b->sched = sc;
     if (!capture) return SW_OK;
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-        const hipError_t le = batch_enqueue(b, s);
-        hipGraph_t g = nullptr;
-        const hipError_t ce = hipStreamEndCapture(s, &g);
-        if (le == hipSuccess && ce == hipSuccess && g) {
-            b->graph = g;
-            if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) b->exec = nullptr;
-        } else if (g) {
-            (void)hipGraphDestroy(g);
-        }
+    // A batch runs from its graph or not at all: a failed capture is an error,
+    // not a quiet fall-back to stream launches.
+    const hipError_t be = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (be != hipSuccess)
+        return fail(e, SW_E_HIP, std::string("batch graph capture could not begin: ") + hipGetErrorString(be));
+    const hipError_t le = batch_enqueue(b, s);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(s, &g);
+    if (le != hipSuccess || ce != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
         (void)hipGetLastError();
+        return fail(e, SW_E_HIP, std::string("batch graph capture failed: ") +
+                                     hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    b->graph = g;
+    const hipError_t ie = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        b->exec = nullptr;
+        return fail(e, SW_E_HIP, std::string("batch graph instantiation failed: ") + hipGetErrorString(ie));
     }
     return SW_OK;
 }
@@ -1053,6 +1097,10 @@ void batch_free(sw_batch *b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->run_ev) {
+        (void)hipEventSynchronize(b->run_ev);
+        (void)hipEventDestroy(b->run_ev);
+    }
     for (void *x : {(void *)b->d_items, (void *)b->d_fused, (void *)b->d_chunks, (void *)b->d_counters,
                     (void *)b->d_exc, (void *)b->d_order, (void *)b->d_partial})
         dev_free(x);
@@ -1117,6 +1165,8 @@ SW_API int sw_batch_create(sw_engine *e, const sw_batch_item *items, uint32_t n_
     SW_HIP(e, dev_upload(&b->d_exc, exc.data(), exc.size(), s));
     if (n_order) SW_HIP(e, hipMalloc((void **)&b->d_order, n_order * sizeof(uint32_t)));
     if (partial) SW_HIP(e, hipMalloc((void **)&b->d_partial, partial * sizeof(double)));
+    int rc0 = scratch_acquire(e, s);             // build_order_into uses the engine's sort scratch
+    if (rc0) return rc0;
     for (uint32_t i = 0; i < n_items; i++) {
         sw::WalkArgs &a = args[i];
         a.exc = b->d_exc + exc_off[i];
@@ -1172,9 +1222,11 @@ SW_API int sw_batch_create(sw_engine *e, const sw_batch_item *items, uint32_t n_
     }
     SW_HIP(e, dev_upload(&b->d_items, singles.data(), singles.size(), s));
     SW_HIP(e, dev_upload(&b->d_fused, fused.data(), fused.size(), s));
+    if ((rc0 = scratch_release(e, s))) return rc0;
     SW_HIP(e, hipStreamSynchronize(s));        // host tables go out of scope
     SW_HIP(e, hipEventCreate(&b->ev0));
     SW_HIP(e, hipEventCreate(&b->ev1));
+    SW_HIP(e, hipEventCreateWithFlags(&b->run_ev, hipEventDisableTiming));
     int rc = apply_schedule(e, b.get(), kDefaultSchedule, s, true);
     if (rc) return rc;
     *out = b.release();
@@ -1186,14 +1238,21 @@ SW_API int sw_batch_run(sw_batch *b, void *stream, sw_walk_stats *per_item, sw_w
     sw_engine *e = b->eng;
     SW_HIP(e, hipSetDevice(b->device));
     hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    if (!b->exec) return fail(e, SW_E_INVALID, "the batch has no graph (its last schedule change failed)");
     const bool stats = per_item || total;
+    if (b->run_stream && b->run_stream != s) SW_HIP(e, hipStreamWaitEvent(s, b->run_ev, 0));
     if (stats) SW_HIP(e, hipEventRecord(b->ev0, s));
-    if (b->exec) SW_HIP(e, hipGraphLaunch(b->exec, s));
-    else SW_HIP(e, batch_enqueue(b, s));
+    SW_HIP(e, hipGraphLaunch(b->exec, s));
+    std::vector<unsigned long long> c;
+    if (stats) {
+        SW_HIP(e, hipEventRecord(b->ev1, s));
+        c.resize(4 * (size_t)b->n_items);
+        SW_HIP(e, hipMemcpyAsync(c.data(), b->d_counters, c.size() * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, s));
+    }
+    SW_HIP(e, hipEventRecord(b->run_ev, s));
+    b->run_stream = s;
     if (!stats) return SW_OK;
-    SW_HIP(e, hipEventRecord(b->ev1, s));
-    std::vector<unsigned long long> c(4 * (size_t)b->n_items);
-    SW_HIP(e, hipMemcpyAsync(c.data(), b->d_counters, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     SW_HIP(e, hipStreamSynchronize(s));
     float ms = 0;
     SW_HIP(e, hipEventElapsedTime(&ms, b->ev0, b->ev1));
@@ -1220,6 +1279,17 @@ SW_API int sw_batch_info(const sw_batch *b, uint32_t *n_walks, uint32_t *n_fused
     if (!b) return SW_E_INVALID;
     if (n_walks) *n_walks = (uint32_t)b->tiles.size();
     if (n_fused_tiles) *n_fused_tiles = b->n_fused_tiles;
+    return SW_OK;
+}
+
+SW_API int sw_batch_graph_info(const sw_batch *b, int32_t *uses_graph, uint32_t *n_nodes) {
+    if (!b) return SW_E_INVALID;
+    if (uses_graph) *uses_graph = b->exec != nullptr;
+    if (n_nodes) {
+        size_t n = 0;
+        if (b->graph && hipGraphGetNodes(b->graph, nullptr, &n) != hipSuccess) n = 0;
+        *n_nodes = (uint32_t)n;
+    }
     return SW_OK;
 }
 
@@ -1254,9 +1324,12 @@ SW_API int sw_batch_tune(sw_batch *b, void *stream, uint32_t max_trials, sw_batc
     auto timed = [&](const sw_batch_schedule &sc, float *ms) -> int {
         int rc = apply_schedule(e, b, sc, s, false);
         if (rc) return rc;
+        if (b->run_stream && b->run_stream != s) SW_HIP(e, hipStreamWaitEvent(s, b->run_ev, 0));
         SW_HIP(e, hipEventRecord(b->ev0, s));
         SW_HIP(e, batch_enqueue(b, s));
         SW_HIP(e, hipEventRecord(b->ev1, s));
+        SW_HIP(e, hipEventRecord(b->run_ev, s));
+        b->run_stream = s;
         SW_HIP(e, hipEventSynchronize(b->ev1));
         SW_HIP(e, hipEventElapsedTime(ms, b->ev0, b->ev1));
         trials++;
@@ -1370,6 +1443,8 @@ SW_API int sw_echo_reduce_device(sw_engine *e, const void *d_M1, int m1_dtype, c
     if (!e || !d_out || (n_spins && (!d_M1 || !d_T))) return fail(e, SW_E_INVALID, "null argument");
     hipStream_t s = stream ? (hipStream_t)stream : e->stream;
     SW_HIP(e, hipSetDevice(e->device));
+    int rc = scratch_acquire(e, s);               // d_partial is engine-owned
+    if (rc) return rc;
     const size_t need = sw::reduce_partial_len(n_spins, n_te, n_sub);
     if (need > e->partial_cap) {
         SW_HIP(e, hipStreamSynchronize(s));
@@ -1380,7 +1455,7 @@ SW_API int sw_echo_reduce_device(sw_engine *e, const void *d_M1, int m1_dtype, c
     }
     SW_HIP(e, sw::launch_echo_reduce(d_M1, m1_dtype == SW_DTYPE_F64, d_T, n_spins, n_te, n_sub, e->d_partial, d_out,
                                      s));
-    return SW_OK;
+    return scratch_release(e, s);
 }
 
 // diagnostic: the production draws (not part of the reference interface)

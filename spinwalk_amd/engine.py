@@ -113,7 +113,7 @@ EXPORTS = (
     "sw_replay_fill", "sw_phantom_create", "sw_phantom_create_device", "sw_phantom_destroy",
     "sw_phantom_info", "sw_phantom_download", "sw_phantom_generate", "sw_sequence_create", "sw_sequence_destroy", "sw_walk_device", "sw_walk_host",
     "sw_batch_create", "sw_batch_run", "sw_batch_destroy", "sw_batch_get_schedule", "sw_batch_set_schedule",
-    "sw_batch_tune", "sw_batch_info",
+    "sw_batch_tune", "sw_batch_info", "sw_batch_graph_info",
     "sw_echo_reduce_device", "sw_philox_draws_device", "sw_probe_gather_device", "sw_stream_sync",
 )
 
@@ -180,6 +180,7 @@ def load():
     L.sw_batch_destroy.argtypes = [_P]
     L.sw_batch_destroy.restype = None
     L.sw_batch_info.argtypes = [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    L.sw_batch_graph_info.argtypes = [_P, C.POINTER(C.c_int32), C.POINTER(C.c_uint32)]
     L.sw_batch_get_schedule.argtypes = [_P, C.POINTER(SwBatchSchedule)]
     L.sw_batch_set_schedule.argtypes = [_P, C.POINTER(SwBatchSchedule)]
     L.sw_batch_tune.argtypes = [_P, _P, C.c_uint32, C.POINTER(SwBatchSchedule), C.POINTER(C.c_float),
@@ -654,6 +655,12 @@ class Batch:
         w, f = C.c_uint32(), C.c_uint32()
         self.eng._check(self.eng.L.sw_batch_info(self.h, C.byref(w), C.byref(f)), "sw_batch_info")
         return dict(walks=w.value, fused_tiles=f.value)
+
+    def graph_info(self):
+        """Whether runs launch the batch's captured HIP graph, and its node count."""
+        g, n = C.c_int32(), C.c_uint32()
+        self.eng._check(self.eng.L.sw_batch_graph_info(self.h, C.byref(g), C.byref(n)), "sw_batch_graph_info")
+        return dict(uses_graph=bool(g.value), nodes=n.value)
 
     @property
     def schedule(self):

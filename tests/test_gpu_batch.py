@@ -98,6 +98,8 @@ def test_batch_equals_separate_walks_over_graph_replays(engine):
     tiles = [_tile(engine, torch, c, si, **kw) for c, si, kw in cs[:-1]]
     refs = [_separate(engine, torch, t) for t in tiles]
     b = engine.batch(tiles)
+    gi = b.graph_info()
+    assert gi["uses_graph"] and gi["nodes"] >= 2, gi      # counter reset + walks (+ echo sums), one graph
     other = _tile(engine, torch, cs[0][0])
     for rep in range(3):
         _clear(tiles)
@@ -267,3 +269,33 @@ def test_empty_inputs(engine):
     assert (empty["echo_sums"].cpu().numpy() == 0).all()
     _check(full, ref, st["items"][1])
     b.close()
+
+
+def test_walks_on_two_streams_share_the_engine_scratch_safely(engine):
+    """sw_walk_device keeps its counters, visiting order and excitation tables in
+    engine-owned buffers.  Two walks enqueued back to back on two streams, the
+    second with more spins (so its visiting order rewrites the buffer the first
+    walk reads), must each equal their walk on the engine's own stream: every
+    spin walked exactly once, counters per call."""
+    import torch
+    m, f = cases.three_substrate_phantom(n=24, fov_um=48.0, seed=51)
+    cfg = cases.se(3, TE=8000, dt=50, P_XY=[[1, 0.5, 0.5], [0.5, 1, 0.5], [0.5, 0.5, 1]])
+    ca = cases.make_case(cfg, m, f, 48.0, n_spins=6000)
+    cb = cases.make_case(dict(cfg, SEED=77), m, f, 48.0, n_spins=24000)
+    ph = engine.phantom(m, f, ca.fov)
+    ta = _tile(engine, torch, ca, ph=ph)
+    tb = _tile(engine, torch, cb, ph=ph)
+    ra, rb = _separate(engine, torch, ta), _separate(engine, torch, tb)
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    for rep in range(2):
+        _clear([ta, tb])
+        torch.cuda.synchronize()
+        outs = []
+        for t, s in ((ta, sa), (tb, sb)):
+            with torch.cuda.stream(s):
+                engine.walk_device(t["phantom"], t["sequence"], t["params"], t["M0"], t["XYZ0"], t["M1"], t["XYZ1"],
+                                   t["T"], spatial_order=1, stream=s, stats=False)
+                engine.echo_reduce(t["M1"], t["T"], t["params"].n_substrate, t["echo_sums"], stream=s)
+        torch.cuda.synchronize()
+        for t, r in ((ta, ra), (tb, rb)):
+            _check(t, r)
