@@ -1,13 +1,22 @@
 #!/usr/bin/env python3
 """Throughput of the SpinWalk `sim` hot path on one MI355X, in spin-steps/s.
 
-Workload (default ``c2``): a 500^3 voxel, 500 um phantom of randomly oriented
-cylinders (r 5 um, 4 %) and spheres (r 3 um, 2 %) generated on the device, 1e6
-spins, spin echo 90/180 at TE 40 ms with dt 20 us (2001 steps per spin),
-permeability 0.5 between the three substrates.  One "step" of this benchmark is
-one complete simulation of that workload (every spin walks every time step)
-followed by the FP64 echo reduction; inputs are resident in HBM when the timed
-region starts.  The value is accepted spin-steps per second.
+Default workload (``c4``), the FoV-scaling sweep: one 256^3 voxel, 256 um phantom
+of randomly oriented cylinders (r 8 um, 4 %) generated on the device, walked at
+32 FoV scales (scaled radii 1-100 um), each with GRE (TE 20 ms) and SE (TE 40 ms)
+and 1e5 spins: 64 tiles whose per-tile time steps give 1 001 to 20 001 steps per
+spin, 3.16e10 spin-steps in all.  One "step" of this benchmark is one complete
+simulation of every tile followed by the FP64 echo reduction of each; inputs
+are resident in HBM when the timed region starts.  `value` is the batched
+engine at the reference's precision (FP32), all tiles in one launch of one
+captured HIP graph.  Beside it, the same tiles walked as the reference launches
+them (one plain FP32 walk per tile in turn) and the batched engine in mixed
+precision (Kahan-compensated FP32 phase, int16 fixed-point field), with every
+signal's error against the FP64 walk of the same draws.
+
+``--workload c1|c2|c3|c5`` selects another workload (c1: 100^3 GRE, 1e5 spins;
+c2: 500^3 spin echo, 1e6 spins, one walk; c3: 1000^3 bSSFP, four phase-cycling
+tiles of 1e7 spins; c5: 64 phantoms).
 
 Prints one JSON line.  ``--gpus N`` with N > 1 is refused: this engine runs on one
 GPU (spins shard trivially across GPUs; that path is not built here).
@@ -79,16 +88,61 @@ def _max_rel(a, ref):
     return max(errs) if errs else None
 
 
+def _brick_morton_order(XYZ0, xyz_scale, s2g, dims):
+    """A spatial visiting order of spins: sorted by the Morton index of the 8^3
+    voxel brick of the start position (the order the engine builds on the
+    device for spatial_order = 1), computed once on the host."""
+    v = np.floor(XYZ0.astype(np.float64) * float(np.float32(xyz_scale)) * np.asarray(s2g)[None, :]).astype(np.int64)
+    v = np.clip(v, 0, np.asarray(dims, np.int64)[None, :] - 1) >> 3
+    key = np.zeros(len(v), np.int64)
+    for bit in range(10):
+        for ax in range(3):
+            key |= ((v[:, ax] >> bit) & 1) << (3 * bit + 2 - ax)
+    return np.argsort(key, kind="stable")
+
+
+def cpu_info():
+    """The host CPU model and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return model, allowed
+
+
+def cpu_threads():
+    """Threads of the CPU baseline: every CPU this process may use, capped by
+    OMP_NUM_THREADS when the machine sets it (the host's share of a shared GPU box)."""
+    _, allowed = cpu_info()
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(allowed, int(cap))) if cap and cap.isdigit() else allowed
+
+
 def main_tiles(args, torch, E, W, eng, stream, dev):
-    """A workload of many tiles (the FoV-scaling sweep): the reference's way, one
-    plain FP32 walk per tile in turn, against one batched launch in the requested
-    precision and field format, each with the FP64 echo sums per tile; the signal
-    error of both against the FP64 walk of the same draws."""
+    """A workload of many tiles (default: the FoV-scaling sweep, 32 scales x GRE/SE
+    of 1e5 spins on one 256^3 phantom).  Three ways of walking the same tiles:
+      - the reference's launch structure: one plain FP32 walk per tile, in turn
+        (src/sim/monte_carlo.cu:271-342 launches one cu_sim per scale);
+      - the batched engine in plain FP32, every tile in one launch of one
+        captured HIP graph: this is `value`;
+      - the batched engine in the mixed precision of --precision/--field
+        (default Kahan-compensated FP32 phase, int16 fixed-point field).
+    The signal errors are against the FP64 batch of the same draws."""
     wl = W.WORKLOADS[args.workload]()
-    fmt, prec = FIELDS[args.field], PRECISIONS[args.precision]
-    log(f"[bench] workload {wl['name']}: generating phantom on the device")
-    phs, cases_, pidx, vf = W.build_tiles(eng, wl, formats=(E.SW_FIELD_F32, fmt))
-    tiles = []
+    prec = PRECISIONS[args.precision or "kahan"]
+    fmt = FIELDS[args.field or "q16"]
+    log(f"[bench] workload {wl['name']}: generating phantoms on the device")
+    phs, cases_, pidx, vf = W.build_tiles(eng, wl, formats=tuple(sorted({E.SW_FIELD_F32, fmt})))
+    tiles, seq_tiles = [], []
     inputs = {}        # one device copy of each distinct start array, shared by the tiles that use it
 
     def resident(a):
@@ -97,22 +151,30 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
             inputs[key] = (a, torch.from_numpy(a).to(dev))
         return inputs[key][1]
 
+    z = lambda *sh, dt=torch.float32: torch.zeros(sh, dtype=dt, device=dev)
     for case, k in zip(cases_, pidx):
         p, xs, gs = case.scale_params(case.scales[0])
         n = case.n_spins
-        z = lambda *sh, dt=torch.float32: torch.zeros(sh, dtype=dt, device=dev)
         tiles.append(dict(phantom=phs[k], sequence=eng.sequence(p, case.tables), params=p, xyz_scale=xs,
                           gradient_scale=gs, spin_offset=0, n_local=n, spatial_order=args.order,
                           M0=resident(case.M0), XYZ0=resident(case.XYZ0),
                           M1=z(n, case.n_te, 3), T=z(n, case.n_te, dt=torch.uint8), XYZ1=z(n, case.traj_len, 3),
                           echo_sums=z(case.n_te, p.n_substrate, 3, dt=torch.float64), steps=case.steps_per_spin()))
+        # The per-tile baseline visits the spins in the same spatial order as the
+        # batch, with the order built here once (the batch builds its orders once,
+        # at creation), so the timed region holds the walks alone.
+        X0 = case.XYZ0
+        if args.order:
+            s2g = [p.phantom_size[i] / p.fov[i] for i in range(3)]
+            X0 = np.ascontiguousarray(X0[_brick_morton_order(X0, xs, s2g, list(p.phantom_size))])
+        seq_tiles.append(dict(tiles[-1], XYZ0=torch.from_numpy(X0).to(dev)))
     total_steps = sum(t["n_local"] * t["steps"] for t in tiles)
 
     def sequential():
-        for t in tiles:
+        for t in seq_tiles:
             eng.walk_device(t["phantom"], t["sequence"], t["params"], t["M0"], t["XYZ0"], t["M1"], t["XYZ1"], t["T"],
                             xyz_scale=t["xyz_scale"], gradient_scale=t["gradient_scale"], precision=0,
-                            field_format=E.SW_FIELD_F32, spatial_order=args.order, stream=stream, stats=False)
+                            field_format=E.SW_FIELD_F32, spatial_order=0, stream=stream, stats=False)
             eng.echo_reduce(t["M1"], t["T"], t["params"].n_substrate, t["echo_sums"], stream=stream)
 
     def timed(fn):
@@ -125,87 +187,124 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / args.steps
 
-    seq_s = timed(sequential)
-    seq_sig = [_signals(t["echo_sums"].cpu().numpy()) for t in tiles]
-    log(f"[bench] one plain FP32 walk per tile: {seq_s * 1e3:.2f} ms, {total_steps / seq_s:.4g} spin-steps/s")
-
     def make_batch(precision, field):
         dt = torch.float64 if precision == 2 else torch.float32
         items = []
         for t in tiles:
             it = dict(t)
             it.update(precision=precision, field_format=field, M1=torch.zeros(tuple(t["M1"].shape), dtype=dt, device=dev),
+                      T=torch.zeros_like(t["T"]), XYZ1=torch.zeros_like(t["XYZ1"]),
                       echo_sums=torch.zeros_like(t["echo_sums"]))
             items.append(it)
         return eng.batch(items), items
 
-    b, items = make_batch(prec, fmt)
-    st = {}
-    tuning = None
-    if args.tune:
-        tuning = b.tune(stream=stream)
-        log(f"[bench] autotuner: {tuning['schedule']}, {tuning['best_ms']:.2f} ms per run "
-            f"(starting schedule {tuning['first_ms']:.2f} ms)")
+    def run_batch(precision, field, label):
+        b, items = make_batch(precision, field)
+        tuning = None
+        if args.tune:
+            tuning = b.tune(stream=stream)
+            log(f"[bench] autotuner ({label}): {tuning['schedule']}, {tuning['best_ms']:.2f} ms per run "
+                f"(starting schedule {tuning['first_ms']:.2f} ms)")
+        st = {}
 
-    def batched():
-        st.update(b.run(stream=stream, stats=True)["total"])
+        def batched():
+            st.update(b.run(stream=stream, stats=True)["total"])
 
-    b_s = timed(batched)
-    b_sig = [_signals(it["echo_sums"].cpu().numpy()) for it in items]
-    log(f"[bench] batched ({args.precision}, {args.field} field): {b_s * 1e3:.2f} ms, "
-        f"{total_steps / b_s:.4g} spin-steps/s, graph time {st['kernel_ms']:.2f} ms, lost {st['n_lost']}")
+        secs = timed(batched)
+        g = b.graph_info()
+        sig = [_signals(it["echo_sums"].cpu().numpy()) for it in items]
+        log(f"[bench] batched {label}: {secs * 1e3:.2f} ms, {total_steps / secs:.4g} spin-steps/s, "
+            f"kernel {st['kernel_ms']:.2f} ms, lost {st['n_lost']}, HIP graph {g}")
+        return dict(secs=secs, stats=dict(st), graph=g, signals=sig, schedule=b.schedule, tuning=tuning,
+                    info=b.info(), batch=b, items=items)
+
+    seq_s = timed(sequential)
+    log(f"[bench] one plain FP32 walk per tile: {seq_s * 1e3:.2f} ms, {total_steps / seq_s:.4g} spin-steps/s")
+    f32 = run_batch(0, E.SW_FIELD_F32, "FP32")
+    mixed = run_batch(prec, fmt, f"{args.precision or 'kahan'} + {args.field or 'q16'} field")
+    for r in (f32, mixed):
+        r["batch"].close()
+        del r["items"]
     b64, items64 = make_batch(2, E.SW_FIELD_F32)
     b64.run(stream=stream, stats=True)
     ref_sig = [_signals(it["echo_sums"].cpu().numpy()) for it in items64]
-    err_b = max((_max_rel(a, r) or 0.0) for a, r in zip(b_sig, ref_sig))
-    err_s = max((_max_rel(a, r) or 0.0) for a, r in zip(seq_sig, ref_sig))
-    per_tile = [dict(tile=i, scale=float(wl["tiles"][i]["scale"]),
-                     signals={f"echo{te}_label{lab}": dict(value=a[(te, lab)], f64=r[(te, lab)],
-                                                           rel_err=abs(a[(te, lab)] - r[(te, lab)]) / r[(te, lab)])
-                              for (te, lab) in r if (te, lab) in a and r[(te, lab)] > 0})
-                for i, (a, r) in enumerate(zip(b_sig, ref_sig))]
-    log(f"[bench] max relative signal error against FP64: batched {err_b:.2e}, plain FP32 {err_s:.2e}")
-    rows = []
-    for i, (a, r) in enumerate(zip(b_sig, ref_sig)):
-        for key in r:
-            if key in a and r[key] > 0:
-                rows.append(dict(tile=i, seq=wl["tiles"][i]["seq"], scale=float(wl["tiles"][i]["scale"]),
-                                 echo=key[0], label=key[1], signal=a[key], f64=r[key],
-                                 abs_err=abs(a[key] - r[key]), rel_err=abs(a[key] - r[key]) / r[key]))
-    worst = sorted(rows, key=lambda x: -x["rel_err"])[:5]
-    for w in worst:
-        log(f"[bench]   tile {w['tile']} {w['seq']} scale {w['scale']:.3g} label {w['label']}: signal "
+    b64.close()
+    del items64
+
+    def errors(sig):
+        rows = []
+        for i, (a, r) in enumerate(zip(sig, ref_sig)):
+            for key in r:
+                if key in a and r[key] > 0:
+                    rows.append(dict(tile=i, seq=wl["tiles"][i]["seq"], scale=float(wl["tiles"][i]["scale"]),
+                                     echo=key[0], label=key[1], signal=a[key], f64=r[key],
+                                     abs_err=abs(a[key] - r[key]), rel_err=abs(a[key] - r[key]) / r[key]))
+        worst = sorted(rows, key=lambda x: -x["rel_err"])[:5]
+        return dict(max_rel=worst[0]["rel_err"] if worst else None,
+                    max_abs=max((w["abs_err"] for w in rows), default=None), worst=worst)
+
+    if args.copy_mib > 0:              # profiler calibration: one device copy of known size
+        a = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=dev)
+        torch.empty_like(a).copy_(a)
+        torch.cuda.synchronize()
+        del a
+    err32, errm = errors(f32["signals"]), errors(mixed["signals"])
+    log(f"[bench] max relative signal error against FP64: FP32 batch {err32['max_rel']:.2e}, "
+        f"mixed batch {errm['max_rel']:.2e}")
+    for w in errm["worst"]:
+        log(f"[bench]   mixed: tile {w['tile']} {w['seq']} scale {w['scale']:.3g} label {w['label']}: signal "
             f"{w['signal']:.6g} vs FP64 {w['f64']:.6g}, abs {w['abs_err']:.2e}, rel {w['rel_err']:.2e}")
-    max_abs = max((w["abs_err"] for w in rows), default=None)
-    bpe = 5 if fmt == E.SW_FIELD_F32 else 3
-    achieved = total_steps * bpe / (st["kernel_ms"] * 1e-3) / 1e9
+
+    def roofline(res, field, tag):
+        bpe = 5 if field == E.SW_FIELD_F32 else 3           # u8 mask + field bytes per spin-step
+        kernel_s = res["stats"]["kernel_ms"] * 1e-3
+        achieved = total_steps * bpe / kernel_s / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{tag}.json")
+        if os.path.exists(tpath):
+            with open(tpath) as fh:
+                traffic = json.load(fh).get("hbm_bytes_per_launch")
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "traffic_rate": None if traffic is None else traffic / kernel_s / 1e9,
+                "bytes_per_spin_step": bpe, "kernel_ms": res["stats"]["kernel_ms"], "kernel": "walk_batch_kernel"}
+
+    mixed_tag = f"{args.precision or 'kahan'}_{args.field or 'q16'}"
     out = {
-        "metric": "spin-steps/s", "value": total_steps / b_s, "unit": "spin-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": b_s * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None,
-        "dtype": {0: "f32", 1: "f32-kahan", 2: "f64"}[prec] + ("" if fmt == 1 else f"+{args.field}-field"),
-        "data": "synthetic",
+        "metric": "spin-steps/s", "value": total_steps / f32["secs"], "unit": "spin-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": f32["secs"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": wl["name"],
                    "phantom": ("x".join(str(d) for d in wl["phantom"]["dims"]) +
                                f" voxels, {wl['phantom']['fov_um'][0]:g} um, random cylinders")
                    if "phantom" in wl else f"{len(phs)} phantoms of cylinders, spheres and packed axons",
                    "tiles": len(tiles), "spins_per_tile": tiles[0]["n_local"], "spin_steps": total_steps,
-                   "precision": args.precision, "field": args.field, "mode": "batched, one HIP graph",
-                   "schedule": b.schedule, "autotuned": tuning is not None, "rng": "philox4x32-10"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_spin_step": bpe,
-                     "kernel_ms": st["kernel_ms"]},
+                   "precision": "f32", "field": "f32",
+                   "mode": "batched" + (", one HIP graph" if f32["graph"]["uses_graph"] else ", stream launches"),
+                   "hip_graph": f32["graph"], "walks": f32["info"], "schedule": f32["schedule"],
+                   "autotuned": f32["tuning"] is not None, "rng": "philox4x32-10"},
+        "roofline": roofline(f32, E.SW_FIELD_F32, "f32_f32"),
+        "signal_error_vs_f64": err32,
         "sequential_f32": {"value": total_steps / seq_s, "ms_per_step": seq_s * 1e3,
-                           "signal_error_vs_f64": err_s},
-        "signal_error_vs_f64": {"max_rel": err_b, "max_abs": max_abs, "worst": worst,
-                                "per_tile": per_tile if len(tiles) <= 8 else None},
+                           "mode": "one plain FP32 walk per tile in turn (the reference's launch structure)"},
+        "mixed_batched": {"value": total_steps / mixed["secs"], "ms_per_step": mixed["secs"] * 1e3,
+                          "precision": args.precision or "kahan", "field": args.field or "q16",
+                          "dtype": {0: "f32", 1: "f32-kahan", 2: "f64"}[prec] + f"+{args.field or 'q16'}-field",
+                          "hip_graph": mixed["graph"], "schedule": mixed["schedule"],
+                          "roofline": roofline(mixed, fmt, mixed_tag), "signal_error_vs_f64": errm,
+                          "speedup_vs_sequential_f32": seq_s / mixed["secs"],
+                          "speedup_vs_batched_f32": f32["secs"] / mixed["secs"]},
     }
-    if tuning is not None:
-        out["autotuner"] = tuning
+    for key, r in (("autotuner", f32), ("autotuner_mixed", mixed)):
+        if r["tuning"] is not None:
+            out[key] = r["tuning"]
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"[bench] CPU baseline: oracle on {threads} threads, {args.cpu_seconds} s budget, first tile")
+        threads = cpu_threads()
+        model, allowed = cpu_info()
+        log(f"[bench] CPU baseline: oracle on {threads} threads ({model}, {allowed} CPUs allowed), "
+            f"{args.cpu_seconds} s budget, first tile")
         out["cpu_baseline"] = cpu_baseline(cases_[0], phs[pidx[0]], args.cpu_seconds, threads)
+        out["cpu_baseline"].update(cpu_model=model, cpus_allowed=allowed, os_cpu_count=os.cpu_count())
     print(json.dumps(out), flush=True)
     return 0
 
@@ -257,9 +356,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2")
-    ap.add_argument("--precision", default="f32", choices=sorted(PRECISIONS))
-    ap.add_argument("--field", default="f32", choices=sorted(FIELDS))
+    ap.add_argument("--workload", default="c4", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--precision", default=None, choices=sorted(PRECISIONS),
+                    help="single-walk workloads: the walk's precision (default f32); tile workloads: the "
+                         "mixed batch's (default kahan)")
+    ap.add_argument("--field", default=None, choices=sorted(FIELDS),
+                    help="single-walk workloads: the field format (default f32); tile workloads: the mixed "
+                         "batch's (default q16)")
     ap.add_argument("--order", type=int, default=1, help="spatial visiting order (1) or spin order (0)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="wall-time budget of the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
@@ -296,6 +399,8 @@ def main():
     if "tiles" in W.WORKLOADS[args.workload]():
         return main_tiles(args, torch, E, W, eng, stream, dev)
     wl = W.WORKLOADS[args.workload]()
+    args.precision = args.precision or "f32"
+    args.field = args.field or "f32"
     fmt = FIELDS[args.field]
     prec = PRECISIONS[args.precision]
     log(f"[bench] workload {wl['name']}: generating phantom on the device")
@@ -405,9 +510,12 @@ def main():
     }
     out["signal_error_vs_f64"] = {"max_rel": max(e["rel_err"] for e in errs) if errs else None, "per_echo_label": errs}
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"[bench] CPU baseline: oracle on {threads} threads, {args.cpu_seconds} s budget")
+        threads = cpu_threads()
+        model, allowed = cpu_info()
+        log(f"[bench] CPU baseline: oracle on {threads} threads ({model}, {allowed} CPUs allowed), "
+            f"{args.cpu_seconds} s budget")
         out["cpu_baseline"] = cpu_baseline(case, ph, args.cpu_seconds, threads)
+        out["cpu_baseline"].update(cpu_model=model, cpus_allowed=allowed, os_cpu_count=os.cpu_count())
     print(json.dumps(out), flush=True)
     ph.close()
     seq.close()

@@ -139,12 +139,21 @@ def pgse_tables(b_first, axis, start_ms, delta_ms, DELTA_ms, dt):
 
 
 def c5():
-    """Heterogeneous batch: 64 phantoms of 128^3-512^3 voxels (1 um voxels) of
-    random cylinders, spheres or packed parallel axons (r 0.5-3 um, 30-70 %), each
-    walked with one of GRE (TE 20 ms), SE (TE 40 ms) or PGSE along x, y or z
-    (delta 10 ms, Delta 30 ms, b = 1000, 0, 2000, 3000 s/mm^2 as gradient scales of
-    the first), with 1e4-1e6 spins (log-uniform), dt 50 us.  Every tile of every
-    phantom goes into one launch."""
+    """Heterogeneous batch: 64 phantoms of 128^3-512^3 voxels (1 um voxels), each
+    of one kind, in turn:
+      - randomly oriented cylinders, r 2-6 um, 2-6 % volume fraction;
+      - spheres, r 2-5 um, 2-6 %;
+      - packed parallel axons, r 0.5-3 um, 30-70 %.
+    The radius 0.5-3 um and the 30-70 % fraction belong to the packed axons:
+    shapes placed at random without overlap (the reference's cylinder and sphere
+    generators, src/phantom/phantom_cylinder.cpp:85-130,
+    src/phantom/phantom_sphere.cpp:79-113) stop fitting far below 30 % (random
+    sequential placement of equal spheres jams near 38 %, of randomly oriented
+    infinite cylinders at a few percent), so only packing reaches that range.
+    Each phantom is walked with one of GRE (TE 20 ms), SE (TE 40 ms) or PGSE along
+    x, y or z (delta 10 ms, Delta 30 ms, b = 1000, 0, 2000, 3000 s/mm^2 as gradient
+    scales of the first), with 1e4-1e6 spins (log-uniform), dt 50 us.  Every tile
+    of every phantom goes into one launch."""
     rng = np.random.default_rng(2024)
     phantoms, tiles = [], []
     for i in range(64):

@@ -150,3 +150,69 @@ def test_generated_phantoms_and_config_run_end_to_end(engine, tmp_path):
         want = (801 * sd) % 360.0
         assert min(abs(d - want), abs((360.0 - want) - d)) < 0.05, (d, want)      # either sense of rotation
     assert step_deg[1] / step_deg[0] == pytest.approx((1 - 0.78) / (1 - 0.85), rel=1e-5)
+
+
+def test_run_config_reads_xyz0_file_and_keeps_the_fov_edge(engine, tmp_path):
+    """Initial positions from an HDF5 `XYZ` dataset (monte_carlo.cu:127-139).  The
+    range check lets a coordinate equal the FoV: on the first axis such a spin's
+    start index lies beyond the volume and the spin is abandoned (its outputs keep
+    the buffer's zeros); on the last axis the index runs into the next row, and the
+    spin walks from that voxel, as the reference's sub2ind makes it."""
+    fov_um, n = 24.0, 64
+    m, f = cases.cylinder_phantom(n=12, fov_um=fov_um, radius_um=3.0, n_cyl=3, seed=31)
+    fov = np.full(3, np.float32(fov_um * 1e-6))
+    H.write_phantom(tmp_path / "ph.h5", m, f, fov)
+    rng = np.random.default_rng(7)
+    xyz0 = (rng.uniform(0.0, 1.0, (n, 3)) * fov).astype(np.float32)
+    xyz0[0] = (fov[0], fov[1] / 2, fov[2] / 2)          # x == fov: abandoned
+    xyz0[1] = (fov[0] / 3, fov[1] / 3, fov[2])          # z == fov: walks from the next row's voxel
+    xyz0[2] = (0.0, 0.0, 0.0)
+    H.h5_write(tmp_path / "xyz.h5", "XYZ", xyz0)
+    cfg = cases.gre(2, TE=2000, dt=50, NUMBER_OF_SPINS=n, DIFFUSIVITY=[1e-9, 1e-9], P_XY=[[1, 0.5], [0.5, 1]])
+    cases.write_config(tmp_path / "g.ini", cfg, ["ph.h5"], output_dir="out", seq_name="gre", xyz0=["xyz.h5"])
+    H.run_config(engine, tmp_path / "g.ini", rng=E.SW_RNG_REPLAY)
+    M = H.h5_read(tmp_path / "out" / "gre_ph.h5", "M")
+    T = H.h5_read(tmp_path / "out" / "gre_ph.h5", "T", np.uint8)
+    X = H.h5_read(tmp_path / "out" / "gre_ph.h5", "XYZ")
+    case = cases.make_case(cfg, m, f, fov_um, xyz0=xyz0)
+    o = orc.sim(cases.to_oracle(case, 0))
+    _check(M[0], T[0][..., 0], X[0], o)
+    assert o["lost"][0] == 1 and o["lost"][1:].sum() == 0
+    assert (M[0, 0] == 0).all() and (T[0, 0] == 0).all()
+    np.testing.assert_array_equal(X[0, 0, 0], xyz0[0])
+    assert np.abs(M[0, 1:, 0, :2]).max() > 0.5
+
+
+@pytest.mark.parametrize("where", ["above", "below"])
+def test_run_config_rejects_xyz0_outside_the_fov(engine, tmp_path, where):
+    fov_um, n = 24.0, 16
+    m, f = cases.cylinder_phantom(n=12, fov_um=fov_um, radius_um=3.0, n_cyl=2, seed=32)
+    fov = np.full(3, np.float32(fov_um * 1e-6))
+    H.write_phantom(tmp_path / "ph.h5", m, f, fov)
+    xyz0 = np.full((n, 3), fov[0] / 2, np.float32)
+    xyz0[5, 1] = np.nextafter(fov[1], np.float32(1)) if where == "above" else np.float32(-1e-12)
+    H.h5_write(tmp_path / "xyz.h5", "XYZ", xyz0)
+    cfg = cases.gre(2, TE=2000, dt=50, NUMBER_OF_SPINS=n)
+    cases.write_config(tmp_path / "g.ini", cfg, ["ph.h5"], output_dir="out", seq_name="gre", xyz0=["xyz.h5"])
+    with pytest.raises(E.EngineError, match="Initial positions are outside the FoV"):
+        H.run_config(engine, tmp_path / "g.ini")
+    assert not (tmp_path / "out" / "gre_ph.h5").exists()
+
+
+def test_run_config_rejects_a_mask_with_more_labels_than_substrates(engine, tmp_path):
+    """read_phantom's check (monte_carlo.cu:113-118): max(mask) + 1 substrates
+    must be described by the config."""
+    fov_um = 24.0
+    m, f = cases.three_substrate_phantom(n=12, fov_um=fov_um, seed=33)
+    assert m.max() == 2
+    H.write_phantom(tmp_path / "ph.h5", m, f, np.full(3, np.float32(fov_um * 1e-6)))
+    cfg = cases.gre(2, TE=2000, dt=50, NUMBER_OF_SPINS=16)
+    cases.write_config(tmp_path / "g.ini", cfg, ["ph.h5"], output_dir="out", seq_name="gre")
+    with pytest.raises(E.EngineError, match="The number of substrate types in the mask does not match the number "
+                                            "of substrate types in the config file: 3 vs 2"):
+        H.run_config(engine, tmp_path / "g.ini")
+    # the same phantom with a third substrate in the config runs
+    cfg3 = cases.gre(3, TE=2000, dt=50, NUMBER_OF_SPINS=16)
+    cases.write_config(tmp_path / "g3.ini", cfg3, ["ph.h5"], output_dir="out", seq_name="gre3")
+    H.run_config(engine, tmp_path / "g3.ini")
+    assert H.h5_read(tmp_path / "out" / "gre3_ph.h5", "M").shape == (1, 16, 1, 3)

@@ -1,7 +1,16 @@
 """Parity and invariants at the benchmark workloads' full sizes.
 
-The oracle cannot walk a million spins in a test's time, so at full size the
-walk is held to what does not depend on size, and to the oracle on a sample:
+100^3 GRE (100^3 voxels, 1e5 spins, 401 steps, impermeable cylinders): the one
+workload the oracle walks whole in a test's time, so every spin is checked:
+  - all 1e5 spins replayed with the reference CPU path's own draw streams:
+    trajectories, labels and lost spins bit-identical to the oracle, M within
+    1e-5 relative, the FP64 path within 1e-12 of the oracle's FP64 twin;
+  - all 1e5 spins with production draws: each label's signal |<Mxy>| agrees
+    with the oracle's (the reference's own draws) within five standard errors
+    of the difference of two independent Monte-Carlo estimates.
+
+For the larger workloads the oracle cannot walk every spin in a test's time, so
+the walk is held to what does not depend on size, and to the oracle on a sample:
 
 500^3 spin echo (500^3 voxels generated on the device, 1e6 spins, 2001 steps,
 three substrates, permeability 0.5):
@@ -64,6 +73,77 @@ def _signals(sums):
 def _max_rel(a, ref):
     assert a.keys() == ref.keys()
     return max(abs(a[k] - ref[k]) / ref[k] for k in ref if ref[k] > 0)
+
+
+@pytest.fixture(scope="module")
+def c1(engine):
+    ph, case, _ = W.build(engine, W.c1(), formats=(E.SW_FIELD_F32,))
+    mask, field = ph.download(field=True)
+    d = orc.from_engine_case(case, 0, mask=mask, field_T=field)
+    yield ph, case, d
+    ph.close()
+
+
+def test_c1_every_spin_replayed_bit_exact(engine, c1):
+    ph, case, d = c1
+    n, steps = case.n_spins, case.steps_per_spin()
+    assert n == 100000 and steps == 401 and case.n_te == 1
+    # impermeable walls: a rejected attempt draws again, so room for three attempts a step
+    normals, uniforms = E.replay_fill(case.params.seed, 0, n, 9 * steps + 96, 3 * steps + 64, n_threads=16)
+    p, xs, gs = case.scale_params(case.scales[0])
+    seq = engine.sequence(p, case.tables)
+    for prec, f64 in ((E.SW_PREC_F32, False), (E.SW_PREC_F64, True)):
+        print(f"[c1] replay all {n} spins, precision {prec}", flush=True)
+        o = orc.sim(d, n_threads=16, f64=f64)
+        M1 = np.zeros((n, case.n_te, 3), np.float64 if f64 else np.float32)
+        T = np.zeros((n, case.n_te), np.uint8)
+        XYZ1 = np.zeros((n, case.traj_len, 3), np.float32)
+        st = engine.walk_host(ph, seq, p, case.M0, case.XYZ0, M1, XYZ1, T, xyz_scale=xs, gradient_scale=gs,
+                              rng=E.SW_RNG_REPLAY, precision=prec, field_format=E.SW_FIELD_F32,
+                              replay=(normals, uniforms), spatial_order=1)
+        _assert_parity(dict(M1=M1, T=T, XYZ1=XYZ1, stats=st), o, f64=f64)
+        if o["n_lost"] == 0:
+            assert st["n_steps"] == n * steps
+        assert st["n_attempts"] > st["n_steps"]           # the walls reject attempts
+    seq.close()
+
+
+def _label_signals(M1, T):
+    """Per label: |<Mxy>| and the standard error of it (the spread of the spins'
+    Mxy along the mean direction over the square root of their number)."""
+    out = {}
+    z = M1[:, 0, 0].astype(np.float64) + 1j * M1[:, 0, 1].astype(np.float64)
+    for lab in np.unique(T[:, 0]):
+        zz = z[T[:, 0] == lab]
+        m = zz.mean()
+        proj = (zz * np.conj(m) / abs(m)).real
+        out[int(lab)] = (abs(m), proj.std(ddof=1) / np.sqrt(zz.size), zz.size)
+    return out
+
+
+def test_c1_production_signals_within_monte_carlo_error_of_oracle(engine, c1):
+    ph, case, d = c1
+    n = case.n_spins
+    o = orc.sim(d, n_threads=16)
+    p, xs, gs = case.scale_params(case.scales[0])
+    seq = engine.sequence(p, case.tables)
+    M1 = np.zeros((n, case.n_te, 3), np.float32)
+    T = np.zeros((n, case.n_te), np.uint8)
+    XYZ1 = np.zeros((n, case.traj_len, 3), np.float32)
+    st = engine.walk_host(ph, seq, p, case.M0, case.XYZ0, M1, XYZ1, T, xyz_scale=xs, gradient_scale=gs,
+                          rng=E.SW_RNG_PHILOX, precision=E.SW_PREC_F32, field_format=E.SW_FIELD_F32,
+                          spatial_order=1)
+    seq.close()
+    assert st["n_lost"] == 0 and o["n_lost"] == 0
+    assert st["n_steps"] == n * case.steps_per_spin()
+    g, r = _label_signals(M1, T), _label_signals(o["M1"], o["T"])
+    assert g.keys() == r.keys() == {0, 1}
+    for lab in r:
+        # impermeable walls keep every spin in its start label
+        assert g[lab][2] == r[lab][2] == int((T[:, 0] == lab).sum())
+        se = np.hypot(g[lab][1], r[lab][1])
+        assert abs(g[lab][0] - r[lab][0]) < 5 * se, (lab, g[lab], r[lab])
+        assert se < 0.05 * r[lab][0]                      # the test resolves each signal to 5 % or better
 
 
 def test_c2_replay_sample_on_the_full_phantom(engine, c2):
