@@ -212,6 +212,27 @@ SW_API int  sw_walk_device(sw_engine *eng, const sw_phantom *ph, const sw_sequen
                            void *d_M1, float *d_XYZ1, uint8_t *d_T,
                            void *stream, sw_walk_stats *stats);
 
+/* sw_walk_device without allocation.  No counterpart in the reference: this
+ * entry point is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as sw_walk_device.  The counters, visiting order and
+ * sort buffers live in a fixed 32 MiB device array that the library holds on
+ * every device it runs on until the process ends; with spatial_order the call
+ * needs 256 + 4*n_local (rounded up to 256) bytes plus the spatial sort's
+ * buffers (8*n_local plus 2 MiB for phantoms up to 512^3, 16 MiB up to
+ * 1024^3), so n_local up to about 2.6 million spins (1.4 million on phantoms
+ * above 512^3), and refuses larger walks with SW_E_INVALID before launching
+ * anything (walks without spatial_order have no limit).  The sequence's excitation table must
+ * match p (sw_sequence_create builds it for the parameters given there).
+ * Every call on one device shares that device's array, including calls of
+ * different engines: calls that overlap in time on one device race and must
+ * be ordered by the caller.  Calls no hipMalloc or hipFree. */
+SW_API int  sw_walk_device_noalloc(sw_engine *eng, const sw_phantom *ph, const sw_sequence *seq,
+                                   const sw_params *p, const sw_walk_opts *o,
+                                   const float *d_M0, const float *d_XYZ0,
+                                   void *d_M1, float *d_XYZ1, uint8_t *d_T,
+                                   void *stream, sw_walk_stats *stats);
+
 /* Host-pointer convenience with the reference's semantics: M1/T/XYZ1 are
  * uploaded first (so untouched entries keep the caller's values), the walk runs,
  * results are copied back; synchronous. */

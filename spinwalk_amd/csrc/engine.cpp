@@ -54,6 +54,10 @@ struct sw_engine {
     // last use; the next call on another stream waits for it first.
     hipEvent_t scratch_ev = nullptr;
     hipStream_t scratch_stream = nullptr;     // stream of the last recorded use
+    // sw_walk_device_noalloc: the device address of this device's fixed scratch
+    // (noalloc.hip), looked up at engine creation; na_err keeps a failed lookup.
+    void *na_scratch = nullptr;
+    hipError_t na_err = hipSuccess;
 };
 
 struct sw_phantom {
@@ -268,6 +272,7 @@ SW_API int sw_engine_create(int device_id, sw_engine **out) {
     if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
         hipEventCreateWithFlags(&e->scratch_ev, hipEventDisableTiming) != hipSuccess)
         return fail(nullptr, SW_E_HIP, "hipEventCreate failed");
+    e->na_err = sw::noalloc_scratch_init(device_id, &e->na_scratch);   // reported by sw_walk_device_noalloc
     *out = e.release();
     return SW_OK;
 }
@@ -848,6 +853,64 @@ SW_API int sw_walk_device(sw_engine *e, const sw_phantom *ph, const sw_sequence 
     }
     if ((rc = scratch_release(e, s))) return rc;
     if (stats) {
+        SW_HIP(e, hipStreamSynchronize(s));
+        float ms = 0;
+        SW_HIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        stats->n_lost = c[0]; stats->n_replay_exhausted = c[1]; stats->n_steps = c[2]; stats->n_attempts = c[3];
+        stats->kernel_ms = ms;
+        if (c[1]) return fail(e, SW_E_REPLAY, std::to_string(c[1]) + " replayed spins ran out of recorded draws");
+    }
+    return SW_OK;
+}
+
+// The walk of sw_walk_device with no allocation: the counters, visiting order
+// and sort buffers sit in the fixed device scratch of noalloc.hip instead of
+// engine-owned memory.  Layout: [0, 256) counters, then the visiting order
+// (4 bytes a spin), then the sort's buffers.  Nothing here allocates device
+// memory; the sequence's excitation table must already match p (it is built
+// by sw_sequence_create for the parameters given there, and by any earlier
+// sw_walk_device call with these phase-cycling parameters).
+SW_API int sw_walk_device_noalloc(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, const sw_params *p,
+                                  const sw_walk_opts *o, const float *d_M0, const float *d_XYZ0, void *d_M1,
+                                  float *d_XYZ1, uint8_t *d_T, void *stream, sw_walk_stats *stats) {
+    if (!e || !ph || !q || !p || !o) return fail(e, SW_E_INVALID, "null argument");
+    hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    SW_HIP(e, hipSetDevice(e->device));
+    if (e->na_err != hipSuccess || !e->na_scratch)
+        return fail(e, SW_E_HIP, std::string("no-allocation scratch unavailable: ") + hipGetErrorString(e->na_err));
+    sw::WalkArgs a;
+    int field = 0;
+    int rc = fill_walk_args(e, ph, q, p, o, d_M0, d_XYZ0, d_M1, d_XYZ1, d_T, a, &field);
+    if (rc) return rc;
+    if (!q->exc_valid || q->key_lin != p->linear_phase_cycling || q->key_quad != p->quadratic_phase_cycling ||
+        q->key_ndummy != p->n_dummy_scan)
+        return fail(e, SW_E_INVALID, "the sequence's excitation table was built for other phase-cycling or dummy-scan "
+                                     "parameters; create the sequence with these parameters");
+    const size_t cap = sw::noalloc_scratch_bytes();
+    char *base = (char *)e->na_scratch;
+    a.exc = q->exc;
+    a.counters = (unsigned long long *)base;
+    a.order = nullptr;
+    if (o->spatial_order && o->n_local > 1) {
+        const sw::OrderPlan plan = sw::order_plan(o->n_local, a.nx, a.ny, a.nz);
+        const size_t order_bytes = ((size_t)o->n_local * 4 + 255) & ~(size_t)255;
+        if (256 + order_bytes + plan.total_bytes > cap)
+            return fail(e, SW_E_INVALID, "n_local = " + std::to_string(o->n_local) + " with spatial_order needs " +
+                                             std::to_string(256 + order_bytes + plan.total_bytes) +
+                                             " bytes of scratch; the no-allocation scratch holds " +
+                                             std::to_string(cap) + " (use spatial_order 0 or sw_walk_device)");
+        uint32_t *order = (uint32_t *)(base + 256);
+        SW_HIP(e, sw::launch_spatial_order(plan, base + 256 + order_bytes, a.XYZ0, a.xyz_scale, o->n_local, a.s2g,
+                                           a.nx, a.ny, a.nz, order, s));
+        a.order = order;
+    }
+    SW_HIP(e, sw::launch_zero_u64(a.counters, 4, s));
+    if (stats) SW_HIP(e, hipEventRecord(e->ev0, s));
+    SW_HIP(e, sw::launch_walk(a, o->rng == SW_RNG_REPLAY ? 1 : 0, field, o->precision, s));
+    if (stats) {
+        unsigned long long c[4];
+        SW_HIP(e, hipEventRecord(e->ev1, s));
+        SW_HIP(e, hipMemcpyAsync(c, a.counters, sizeof c, hipMemcpyDeviceToHost, s));
         SW_HIP(e, hipStreamSynchronize(s));
         float ms = 0;
         SW_HIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));

@@ -53,6 +53,12 @@ hipError_t launch_echo_reduce(const void *M1, int f64, const uint8_t *T, uint32_
                               uint32_t n_te, uint32_t n_sub, double *partial, double *out,
                               hipStream_t s);
 
+// The fixed scratch of the no-allocation walk (noalloc.hip): its size, and the
+// device address of `device`'s copy, looked up once per device (`device` must
+// be current on the first call for it).
+size_t noalloc_scratch_bytes();
+hipError_t noalloc_scratch_init(int device, void **addr);
+
 // Clears n counters with a fill kernel (no memset node, safe under graph capture).
 hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
 

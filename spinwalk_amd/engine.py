@@ -111,7 +111,7 @@ EXPORTS = (
     "sw_engine_create", "sw_engine_destroy", "sw_last_error", "sw_device_info", "sw_device_count",
     "sw_version", "sw_params_prepare", "sw_timing_scale", "sw_tesla2deg", "sw_init_positions",
     "sw_replay_fill", "sw_phantom_create", "sw_phantom_create_device", "sw_phantom_destroy",
-    "sw_phantom_info", "sw_phantom_download", "sw_phantom_generate", "sw_sequence_create", "sw_sequence_destroy", "sw_walk_device", "sw_walk_host",
+    "sw_phantom_info", "sw_phantom_download", "sw_phantom_generate", "sw_sequence_create", "sw_sequence_destroy", "sw_walk_device", "sw_walk_device_noalloc", "sw_walk_host",
     "sw_batch_create", "sw_batch_run", "sw_batch_destroy", "sw_batch_get_schedule", "sw_batch_set_schedule",
     "sw_batch_tune", "sw_batch_info", "sw_batch_graph_info",
     "sw_echo_reduce_device", "sw_philox_draws_device", "sw_probe_gather_device", "sw_stream_sync",
@@ -168,6 +168,7 @@ def load():
     L.sw_sequence_destroy.restype = None
     L.sw_walk_device.argtypes = [_P, _P, _P, C.POINTER(SwParams), C.POINTER(SwWalkOpts), _P, _P, _P, _P, _P,
                                  _P, C.POINTER(SwWalkStats)]
+    L.sw_walk_device_noalloc.argtypes = L.sw_walk_device.argtypes
     L.sw_walk_host.argtypes = [_P, _P, _P, C.POINTER(SwParams), C.POINTER(SwWalkOpts), _P, _P, _P, _P, _P,
                                _P, _P, C.POINTER(SwWalkStats)]
     L.sw_echo_reduce_device.argtypes = [_P, _P, C.c_int, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]
@@ -531,8 +532,10 @@ class Engine:
 
     def walk_device(self, ph, seq, params, M0, XYZ0, M1, XYZ1, T, *, spin_offset=0, n_local=None, xyz_scale=1.0,
                     gradient_scale=1.0, rng=SW_RNG_PHILOX, precision=SW_PREC_F32, field_format=SW_FIELD_F32,
-                    spatial_order=0, replay=None, stream=None, stats=True):
-        """torch device tensors; launches on `stream` (torch stream or None)."""
+                    spatial_order=0, replay=None, stream=None, stats=True, noalloc=False):
+        """torch device tensors; launches on `stream` (torch stream or None).
+        noalloc: walk through sw_walk_device_noalloc (fixed device scratch, no
+        allocation) instead of sw_walk_device."""
         o = SwWalkOpts()
         o.spin_offset = int(spin_offset)
         o.n_local = int(M0.shape[0] if n_local is None else n_local)
@@ -549,9 +552,10 @@ class Engine:
         st = SwWalkStats()
         sp, wait = _stream_arg(stream)
         ptr = lambda t: None if t is None else _P(t.data_ptr())
-        rc = self.L.sw_walk_device(self.h, ph.h, seq.h, C.byref(params), C.byref(o), ptr(M0), ptr(XYZ0),
-                                   ptr(M1), ptr(XYZ1), ptr(T), sp, C.byref(st) if stats else None)
-        self._check(rc, "sw_walk_device")
+        fn = self.L.sw_walk_device_noalloc if noalloc else self.L.sw_walk_device
+        rc = fn(self.h, ph.h, seq.h, C.byref(params), C.byref(o), ptr(M0), ptr(XYZ0), ptr(M1), ptr(XYZ1), ptr(T), sp,
+                C.byref(st) if stats else None)
+        self._check(rc, "sw_walk_device_noalloc" if noalloc else "sw_walk_device")
         if wait:
             self.sync()
         return st.as_dict() if stats else None
