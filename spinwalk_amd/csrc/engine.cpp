@@ -789,7 +789,12 @@ int fill_walk_args(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, con
     for (int i = 0; i < 3; i++) {
         a.fov[i] = p->fov[i];
         a.s2g[i] = (double)p->phantom_size[i] / p->fov[i];       // kernels.cu:98
+        a.s2gf[i] = (float)a.s2g[i];
+        a.vs[i] = p->fov[i] / (double)p->phantom_size[i];
+        a.vsf[i] = (float)a.vs[i];
+        a.nf[i] = (float)p->phantom_size[i];
     }
+    a.g2d = (float)(1e-3 * p->timestep_us * 1e-6 * kGamma * kRad2Deg);
     a.diffusivity = q->diffusivity; a.pXY = q->pXY; a.T1s = q->T1s; a.T2s = q->T2s; a.T1d = q->T1d; a.T2d = q->T2d;
     a.n_sub = q->n_sub;
     a.rf = q->rf; a.n_rf = q->n_rf;

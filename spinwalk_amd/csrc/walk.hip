@@ -1,975 +1,13 @@
-// walk.hip -- the per-spin Monte-Carlo walk for gfx950 (MI355X).
-//
-// One lane walks one spin through the whole sequence, as the reference kernel
-// does (src/sim/kernels.cu:55-63 maps thread -> spin and runs sim()), but the
-// state is laid out for CDNA4: positions stay in FP64 registers (full-rate FP64
-// VALU on CDNA4, and FP64 is what the reference's trajectory arithmetic is,
-// kernels.cu:93,131-136), event times are held as "next event" registers so a
-// step costs four integer compares, per-event trigonometry is precomputed per
-// sequence, and the per-step work is: draw -> move -> index -> (gather on voxel
-// change) -> phase add.
-//
-// The step body follows src/sim/kernels.cu:72-233 expression for expression
-// (float vs double promotion, truncation to int64 in sub2ind, the order of
-// events within a step), so with replayed draws the trajectories, tissue labels
-// and phases are the reference CPU path's bit for bit.  This file is compiled
-// with -ffp-contract=off: a fused multiply-add would change roundings.
-#include <hip/hip_runtime.h>
-#include <hip/hip_fp16.h>
-#include <math.h>
-#include <type_traits>
-
-#include "fmod360.hpp"
-#include "philox.hpp"
-#include "walk_args.hpp"
-#include "walk_launch.hpp"
+// walk.hip -- entry points of the walk kernels: dispatch to the kernel families
+// (walk_replay.hip, walk_philox.hip, walk_batch_philox.hip), the index mode of a
+// phantom, the counter fill kernel and the materialised production draws.
+#include "walk_impl.hpp"
 
 namespace sw {
-namespace {
-
-constexpr double kDeg2Rad = 0.0174532925199433;   // simulation_parameters.cuh:24
-constexpr double kRad2Deg = 57.2957795130823;     // simulation_parameters.cuh:25
-constexpr double kGamma   = 267515315.;           // definitions.h:20
-constexpr uint32_t kNever = 0xFFFFFFFFu;
-
-enum : int { ST_OK = 0, ST_LOST = 1, ST_EXHAUSTED = 2 };
-
-
-
-// ---------------------------------------------------------------------------
-// Rotations (src/sim/kernels.cuh:82-152), generic over float/double.
-// ---------------------------------------------------------------------------
-template <class R>
-__device__ __forceinline__ void rot_apply(int kind, R s, R c, R sp, R cp, R m[3]) {
-    R a0 = m[0], a1 = m[1], a2 = m[2];
-    switch (kind) {
-    case ROT_X:    m[1] = c * a1 - s * a2;  m[2] = s * a1 + c * a2;  return;          // xrot(s,c)
-    case ROT_XNEG: m[1] = c * a1 - (-s) * a2; m[2] = (-s) * a1 + c * a2; return;      // xrot(-s,c)
-    case ROT_Y:    m[0] = c * a0 + s * a2;  m[2] = -s * a0 + c * a2; return;          // yrot(s,c)
-    case ROT_YNEG: m[0] = c * a0 + (-s) * a2; m[2] = -(-s) * a0 + c * a2; return;     // yrot(-s,c)
-    default: {
-        // zrot(-sp,cp) -> xrot(s,c) -> zrot(sp,cp)   (kernels.cuh:183-188)
-        const R nsp = -sp;
-        R b0 = cp * a0 - nsp * a1, b1 = nsp * a0 + cp * a1, b2 = a2;
-        R c0 = b0, c1 = c * b1 - s * b2, c2 = s * b1 + c * b2;
-        m[0] = cp * c0 - sp * c1;
-        m[1] = sp * c0 + cp * c1;
-        m[2] = c2;
-        return;
-    }
-    }
-}
-
-// dephase_relax (kernels.cu:45-52): zrot(phase deg) then relax(exp(-t/T1), exp(-t/T2)).
-__device__ __forceinline__ void dephase_relax(float m[3], float phase, float T1, float T2, float t) {
-    const float arg = (float)(phase * kDeg2Rad);            // float * double -> float argument
-    const float s = sinf(arg), c = cosf(arg);
-    const float a0 = m[0], a1 = m[1];
-    m[0] = c * a0 - s * a1;
-    m[1] = s * a0 + c * a1;
-    if (T1 >= 0 && T2 >= 0) {
-        // unqualified exp on a float argument resolves to double exp (kernels.cu:51)
-        const float e1 = (float)exp((double)(-t / T1));
-        const float e2 = (float)exp((double)(-t / T2));
-        m[0] = m[0] * e2;                                    // relax (kernels.cuh:214-219)
-        m[1] = m[1] * e2;
-        m[2] = (float)(1. + e1 * (m[2] - 1.));
-    }
-}
-__device__ __forceinline__ void dephase_relax(double m[3], double phase, double T1, double T2, double t) {
-    const double arg = phase * kDeg2Rad;
-    const double s = sin(arg), c = cos(arg);
-    const double a0 = m[0], a1 = m[1];
-    m[0] = c * a0 - s * a1;
-    m[1] = s * a0 + c * a1;
-    if (T1 >= 0 && T2 >= 0) {
-        const double e1 = exp(-t / T1), e2 = exp(-t / T2);
-        m[0] *= e2; m[1] *= e2; m[2] = 1. + e1 * (m[2] - 1.);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Phase accumulators.
-// ---------------------------------------------------------------------------
-struct PhaseF32 {          // reference: float accumulated_phase (kernels.cu:126,171)
-    using R = float;
-    float v = 0.f;
-    __device__ void add(float x) { v += x; }
-    __device__ void add_grad(double g) { v = (float)((double)v + g); }   // kernels.cu:185
-    __device__ float value() const { return v; }
-    __device__ float angle() const { return v; }     // what the rotation at an event uses
-    __device__ void reset() { v = 0.f; }
-};
-struct PhaseKahan {        // FP32 with Kahan compensation
-    using R = float;
-    float v = 0.f, c = 0.f;
-    __device__ void add(float x) {
-        const float y = x - c;
-        const float t = v + y;
-        c = (t - v) - y;
-        v = t;
-    }
-    __device__ void add_grad(double g) { add((float)g); }
-    __device__ float value() const { return v - c; }
-    // The rotation angle reduced modulo 360 before the compensation is applied
-    // (an exact remainder, fmod360), so a phase of thousands of degrees keeps
-    // its low bits.
-    __device__ float angle() const { return fmod360(v) - c; }
-    __device__ void reset() { v = 0.f; c = 0.f; }
-};
-struct PhaseF64 {
-    using R = double;
-    double v = 0.;
-    __device__ void add(double x) { v += x; }
-    __device__ void add_grad(double g) { v += g; }
-    __device__ double value() const { return v; }
-    __device__ double angle() const { return v; }
-    __device__ void reset() { v = 0.; }
-};
-
-// ---------------------------------------------------------------------------
-// Voxel records.  The reference multiplies the fieldmap by the T->deg/step factor
-// in float once on the host (monte_carlo.cu:243-244); multiplying the stored
-// field_T by the same float factor at the gather gives the same float.
-// ---------------------------------------------------------------------------
-template <class T, bool IDX32>
-__device__ __forceinline__ T load_rec(const void *base, uint64_t i) {
-    if (IDX32) return *(const T *)((const char *)base + (uint32_t)((uint32_t)i * (uint32_t)sizeof(T)));
-    return ((const T *)base)[i];
-}
-
-struct RecF32 {                          // {float field_T, uint32 label}
-    static constexpr BrickShape shape = kBrick8;
-    using T = uint2;
-    __device__ static uint8_t label(T r) { return (uint8_t)r.y; }
-    template <class R> __device__ static R field(const WalkArgs &a, T r) {
-        if (sizeof(R) == 8) return (R)((double)__uint_as_float(r.x) * a.t2d_f64);
-        return (R)(__uint_as_float(r.x) * a.t2d);
-    }
-};
-struct RecNone {                         // label only: {0, uint8 label, pad}
-    static constexpr BrickShape shape = kBrick4;
-    using T = uint32_t;
-    __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
-    template <class R> __device__ static R field(const WalkArgs &, T) { return (R)0; }
-};
-struct RecF16 {                          // {fp16 field_T/half_scale, uint8 label, pad}
-    static constexpr BrickShape shape = kBrick4;
-    using T = uint32_t;
-    __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
-    template <class R> __device__ static R field(const WalkArgs &a, T r) {
-        return (R)(__half2float(__ushort_as_half((unsigned short)(r & 0xFFFFu))) * a.h2d);
-    }
-};
-struct RecQ16 {                          // {int16 rint(field_T/q_scale), uint8 label, pad}
-    static constexpr BrickShape shape = kBrick4;
-    using T = uint32_t;
-    __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
-    template <class R> __device__ static R field(const WalkArgs &a, T r) {
-        return (R)((float)(int32_t)(int16_t)(uint16_t)(r & 0xFFFFu) * a.h2d);
-    }
-};
-struct RecBF16 {                         // {bf16 field_T/half_scale, uint8 label, pad}
-    static constexpr BrickShape shape = kBrick4;
-    using T = uint32_t;
-    __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
-    template <class R> __device__ static R field(const WalkArgs &a, T r) {
-        return (R)(__uint_as_float(r << 16) * a.h2d);
-    }
-};
-
-// ---------------------------------------------------------------------------
-// Draw sources.
-// ---------------------------------------------------------------------------
-// Both sources are used as peek/commit: peek() makes the normals of the next
-// attempt available ahead of time (so they can be computed while the current
-// attempt's gathers are in flight); commit() consumes them.  An attempt that does
-// not move (diffusivity 0) consumes nothing, as in the reference (kernels.cu:130).
-struct ReplayRng {          // the reference CPU path's recorded streams
-    const float *n, *u;
-    uint64_t nn, nu, in = 0, iu = 0;
-    float px = 0.f, py = 0.f, pz = 0.f;
-    bool avail = false;
-    __device__ ReplayRng(const WalkArgs &a, uint32_t li, uint32_t) {
-        n = a.rn + (uint64_t)li * a.rn_stride; nn = a.rn_stride;
-        u = a.ru + (uint64_t)li * a.ru_stride; nu = a.ru_stride;
-    }
-    __device__ void peek() {
-        avail = in + 3 <= nn;
-        if (avail) { px = n[in]; py = n[in + 1]; pz = n[in + 2]; }
-    }
-    __device__ void commit() { in += 3; }
-    __device__ bool uniform(float &v) {       // gen_u stream, one draw per substrate change
-        if (iu >= nu) return false;
-        v = u[iu++];
-        return true;
-    }
-};
-
-struct PhiloxRng {          // production: Philox4x32-10, one block per attempted step
-    uint32_t k0, k1, spin;
-    uint64_t attempt = 0;
-    float px = 0.f, py = 0.f, pz = 0.f, pu = 0.f, ucur = 0.f;
-    static constexpr bool avail = true;
-    __device__ PhiloxRng(const WalkArgs &a, uint32_t, uint32_t spin_global)
-        : k0((uint32_t)a.seed), k1((uint32_t)(a.seed >> 32)), spin(spin_global) {}
-    __device__ void peek() {
-        const U32x4 r = philox4x32_10((uint32_t)attempt, (uint32_t)(attempt >> 32), spin, 0u, k0, k1);
-        // Box-Muller on (r0,r1) and (r2,r3); v_log_f32 is log2, v_sin/v_cos take revolutions
-        const float u1 = u23_open1(r.v[0]), v1 = u23_closed0(r.v[1]);
-        const float u2 = u23_open1(r.v[2]), v2 = u23_closed0(r.v[3]);
-        const float rho1 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
-        const float rho2 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u2));
-        px = rho1 * __builtin_amdgcn_cosf(v1);
-        py = rho1 * __builtin_amdgcn_sinf(v1);
-        pz = rho2 * __builtin_amdgcn_cosf(v2);
-        pu = u24_lowbytes(r.v[0], r.v[1], r.v[2]);
-    }
-    __device__ void commit() { ucur = pu; ++attempt; }
-    // the permeability uniform of the attempt last committed
-    __device__ bool uniform(float &v) { v = ucur; return true; }
-};
-
-// ---------------------------------------------------------------------------
-// sub2ind on truncated scaled positions (kernels.cu:140, kernels.cuh:53-60), and
-// the record of that voxel in the bricked layout.
-// When every scaled coordinate lies in [0, n) the truncations are the subscripts
-// and the voxel is in range; its record index comes from per-axis tables in LDS
-// (IDX32: every record byte offset < 2^32 and nx+ny+nz tables within 64 KiB) or
-// from 64-bit arithmetic.  Anything else takes the reference's int64 arithmetic,
-// including its behaviour for a subscript outside its axis (the index may still
-// land inside the volume, in a neighbouring row); such a voxel is found again
-// from the linear index.
-// ---------------------------------------------------------------------------
-// A voxel lookup: ok when the reference's row-major index lies in
-// [0, matrix_length); rec is then the voxel's record index in the bricked
-// layout, a bijection of the row-major index, so records compare as voxels do.
-template <bool IDX32> using RecIdx = typename std::conditional<IDX32, uint32_t, uint64_t>::type;
-template <bool IDX32> struct Vox {
-    bool ok;
-    RecIdx<IDX32> rec;
-};
-
-// The record index is separable: rec = X(ix) + Y(iy) + Z(iz) with
-//   X = (bx * nb_y * nb_z) << L | (ix mod 2^lbx) << (lby + lbz)
-//   Y = (by * nb_z) << L        | (iy mod 2^lby) << lbz
-//   Z = bz << L                 |  iz mod 2^lbz
-// (b = subscript >> lb, L = lbx + lby + lbz; the parts occupy disjoint bits, so
-// the OR is a sum).  In the 32-bit mode each block keeps the three tables in LDS:
-// axis_tab[0, nx) = X, [nx, nx+ny) = Y, [nx+ny, nx+ny+nz) = Z.
-extern __shared__ uint32_t axis_tab[];
-
-template <class Rec>
-__device__ __forceinline__ uint32_t axis_entry(const WalkArgs &a, uint32_t i) {
-    constexpr int LX = Rec::shape.lbx, LY = Rec::shape.lby, LZ = Rec::shape.lbz, L = LX + LY + LZ;
-    const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny;
-    if (i < nx)
-        return (((i >> LX) * a.nb_y * a.nb_z) << L) | ((i & ((1u << LX) - 1)) << (LY + LZ));
-    if (i < nx + ny) {
-        i -= nx;
-        return (((i >> LY) * a.nb_z) << L) | ((i & ((1u << LY) - 1)) << LZ);
-    }
-    i -= nx + ny;
-    return ((i >> LZ) << L) | (i & ((1u << LZ) - 1));
-}
-
-template <class Rec, bool IDX32>
-__device__ __forceinline__ uint64_t record_of(const WalkArgs &a, uint64_t ix, uint64_t iy, uint64_t iz) {
-    constexpr int LX = Rec::shape.lbx, LY = Rec::shape.lby, LZ = Rec::shape.lbz;
-    if (IDX32) {
-        const uint32_t nx = (uint32_t)a.nx, nxy = nx + (uint32_t)a.ny;
-        return axis_tab[(uint32_t)ix] + axis_tab[nx + (uint32_t)iy] + axis_tab[nxy + (uint32_t)iz];
-    }
-    const uint64_t inner = ((ix & ((1u << LX) - 1)) << (LY + LZ)) | ((iy & ((1u << LY) - 1)) << LZ) |
-                           (iz & ((1u << LZ) - 1));
-    const uint64_t b = ((ix >> LX) * a.nb_y + (iy >> LY)) * a.nb_z + (iz >> LZ);
-    return (b << (LX + LY + LZ)) | inner;
-}
-
-// Truncating double -> int32 conversion with the instruction's defined
-// saturation (a C cast out of range would be undefined).
-__device__ __forceinline__ uint32_t cvt_i32_sat(double v) {
-    uint32_t r;
-    asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(v));
-    return r;
-}
-
-template <class Rec, bool IDX32>
-__device__ __forceinline__ Vox<IDX32> locate(const WalkArgs &a, const double x[3]) {
-    const double vx = x[0] * a.s2g[0], vy = x[1] * a.s2g[1], vz = x[2] * a.s2g[2];
-    Vox<IDX32> v;
-    // v_cvt_i32_f64 truncates toward zero and saturates, so (unsigned)trunc(v) < n
-    // holds exactly when the reference's int64 truncation lies in [0, n).  (It
-    // maps NaN to 0; positions stay finite since every draw is finite.)
-    const uint32_t ix = cvt_i32_sat(vx), iy = cvt_i32_sat(vy), iz = cvt_i32_sat(vz);
-    if (ix < (uint32_t)a.nx && iy < (uint32_t)a.ny && iz < (uint32_t)a.nz) {
-        v.ok = true;
-        v.rec = (RecIdx<IDX32>)record_of<Rec, IDX32>(a, ix, iy, iz);
-        return v;
-    }
-    v.ok = false;
-    v.rec = 0;
-    constexpr double kLim = 9.2e18;   // beyond int64 the conversion is undefined; treat as outside
-    if (!(fabs(vx) < kLim && fabs(vy) < kLim && fabs(vz) < kLim)) return v;
-    const int64_t jx = (int64_t)vx, jy = (int64_t)vy, jz = (int64_t)vz;
-    const int64_t ind = jx * a.nz * a.ny + jy * a.nz + jz;     // kernels.cuh:53-60
-    if (ind >= 0 && ind < a.matrix_length) {                    // kernels.cu:141
-        const uint64_t u = (uint64_t)ind, nz = (uint64_t)a.nz, ny = (uint64_t)a.ny;
-        v.ok = true;
-        v.rec = (RecIdx<IDX32>)record_of<Rec, false>(a, u / (nz * ny), (u / nz) % ny, u % nz);
-    }
-    return v;
-}
-
-// The boundary correction of one axis (kernels.cu:133-136) for v = xo + rw:
-// v < 0: v + adj; v >= fov: v - adj, which rounds as v + (-adj); else v.
-template <bool CROSS>
-__device__ __forceinline__ double fix_axis(double fov, double rw, double v) {
-    const double adj = CROSS ? fov : 2 * fabs(rw);
-    const double sadj = v < 0 ? adj : -adj;
-    return (v < 0 || v >= fov) ? v + sadj : v;
-}
-
-// v < 0 or v >= fov, or v is -0.0 (fov > 0 finite): the bit patterns of the
-// non-negative doubles, read as unsigned integers, are ordered as the values,
-// and every pattern with the sign bit set lies above them.  One 64-bit compare
-// instead of two FP64 compares; fix_axis makes the exact decision.
-__device__ __forceinline__ uint32_t maybe_outside(double v, double fov) {
-    return (uint64_t)__double_as_longlong(v) >= (uint64_t)__double_as_longlong(fov) ? 1u : 0u;
-}
-
-// A step of all three axes (kernels.cu:130-137).  Spins rarely reach a wall, so
-// the correction runs only when some lane of the wave may need it.
-template <bool CROSS>
-__device__ __forceinline__ void move(const WalkArgs &a, float px, float py, float pz, double diff, const double xo[3],
-                                     double xn[3]) {
-    const double rw0 = (double)px * diff, rw1 = (double)py * diff, rw2 = (double)pz * diff;   // :131
-    xn[0] = xo[0] + rw0;
-    xn[1] = xo[1] + rw1;
-    xn[2] = xo[2] + rw2;
-    const bool out = (maybe_outside(xn[0], a.fov[0]) | maybe_outside(xn[1], a.fov[1]) |
-                      maybe_outside(xn[2], a.fov[2])) != 0;
-    if (__builtin_expect(__ballot(out) != 0, 0)) {
-        xn[0] = fix_axis<CROSS>(a.fov[0], rw0, xn[0]);
-        xn[1] = fix_axis<CROSS>(a.fov[1], rw1, xn[1]);
-        xn[2] = fix_axis<CROSS>(a.fov[2], rw2, xn[2]);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// One spin.  Returns ST_OK / ST_LOST / ST_EXHAUSTED; counts accepted steps and
-// rejected attempts.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t next_event(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
-    return min(min(a, b), min(c, d));
-}
-
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-    // 64-lane butterfly; carries folded after each level
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t olo = __shfl_xor(lo, o), ohi = __shfl_xor(hi, o);
-        const uint32_t nlo = lo + olo;
-        hi = hi + ohi + (nlo < lo ? 1u : 0u);
-        lo = nlo;
-    }
-    return ((uint64_t)hi << 32) | lo;
-}
-
-template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
-__device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32_t &n_rej) {
-    using R = typename Phase::R;
-    const uint32_t spin = a.spin_offset + li;
-    float *xyz1 = a.XYZ1 + 3 * (uint64_t)li * a.traj_len;
-    double xo[3], xn[3];
-    R m[3];
-    for (int i = 0; i < 3; i++) {                                  // kernels.cu:95-99
-        const float x0 = a.XYZ0[3 * (uint64_t)li + i] * a.xyz_scale;
-        xo[i] = xn[i] = (double)x0;
-        m[i] = (R)a.M0[3 * (uint64_t)li + i];
-        if (a.record_traj) xyz1[i] = x0;
-    }
-    Vox<IDX32> vx = locate<Rec, IDX32>(a, xo);                     // kernels.cu:102
-    if (!vx.ok) {                                                  // the reference reads out of bounds here
-        if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
-        return ST_LOST;
-    }
-    uint8_t ts_old = Rec::label(load_rec<typename Rec::T, IDX32>(a.rec, vx.rec));   // :103
-    double diff = a.diffusivity[ts_old];                           // :104
-    // The reference re-reads T1/T2/diffusivity of ts_old at every voxel change
-    // (:167-169); they only change with ts_old, so they are read when it changes.
-    // Before the first accepted step T1 = T2 = 0 (:91); no event can use them
-    // before that step unless the TR has no time points at all.
-    R field = 0, T1 = 0, T2 = 0;
-    if (a.n_timepoints > 0) {
-        if (sizeof(R) == 8) { T1 = (R)a.T1d[ts_old]; T2 = (R)a.T2d[ts_old]; }
-        else                { T1 = (R)a.T1s[ts_old]; T2 = (R)a.T2s[ts_old]; }
-    }
-    uint32_t itr = 0;
-    Rng rng(a, li, spin);
-    rng.peek();
-    Phase ph;
-    uint64_t slot = 0;
-    // Every early end (lost, stuck or out of draws) leaves through `done`, so the
-    // step loop has one exit edge besides its natural end.
-    int st = ST_OK;
-    uint32_t d = 0, tp = 0;
-
-    for (d = 0; d <= a.n_dummy; ++d) {                              // :107
-        const bool last = (d == a.n_dummy);
-        {                                                          // :110-120 excitation
-            const Excitation e = a.exc[d];
-            if (sizeof(R) == 8)
-                rot_apply<R>(e.kind_d, (R)a.exc_sd, (R)a.exc_cd, (R)e.spd, (R)e.cpd, m);
-            else
-                rot_apply<R>(e.kind, (R)a.exc_s, (R)a.exc_c, (R)e.sp, (R)e.cp, m);
-        }
-        RecIdx<IDX32> rec_old = ~(RecIdx<IDX32>)0;                 // :123-126 (ind_old: no voxel)
-        uint32_t old_tp = 0;
-        tp = 0;
-        uint32_t rf_i = 1, te_i = 0, dp_i = 0, gr_i = 0;
-        uint32_t next_rf = (a.n_rf > 1) ? (uint32_t)a.rf[1].t : kNever;
-        uint32_t next_te = (last && a.n_te > 0) ? (uint32_t)a.TE_t[0] : kNever;
-        uint32_t next_dp = (a.n_deph > 0) ? (uint32_t)a.deph_t[0] : kNever;
-        uint32_t next_gr = (a.n_grad > 0) ? (uint32_t)a.grad_t[0] : kNever;
-        uint32_t next_ev = next_event(next_rf, next_te, next_dp, next_gr);
-        ph.reset();
-
-        while (tp < a.n_timepoints) {                              // :128
-            const bool moved = diff != 0.;
-            if (moved) {                                           // :130-137
-                if (!rng.avail) { st = ST_EXHAUSTED; goto done; }
-                move<CROSS>(a, rng.px, rng.py, rng.pz, diff, xo, xn);
-            }
-            vx = locate<Rec, IDX32>(a, xn);                        // :140
-            if (!vx.ok) { st = ST_LOST; goto done; }               // :141-147
-            const bool newvox = vx.rec != rec_old;                 // :150
-            typename Rec::T r{};
-            if (newvox)                                            // one gather: label and field
-                r = load_rec<typename Rec::T, IDX32>(a.rec, vx.rec);
-            if (moved) {                                           // next attempt's draws while the load flies
-                rng.commit();
-                rng.peek();
-            }
-            if (newvox) {
-                const uint8_t ts = Rec::label(r);                  // :152
-                const R fnew = Rec::template field<R>(a, r);       // :166, read speculatively
-                if (ts != ts_old) {                                // :153-163
-                    float u;
-                    if (!rng.uniform(u)) { st = ST_EXHAUSTED; goto done; }
-                    if (u >= a.pXY[ts_old * a.n_sub + ts]) {
-                        ++n_rej;
-                        if (itr++ > a.max_iterations) { st = ST_LOST; goto done; }
-                        continue;
-                    }
-                    ts_old = ts;
-                    if (sizeof(R) == 8) { T1 = (R)a.T1d[ts]; T2 = (R)a.T2d[ts]; }
-                    else                { T1 = (R)a.T1s[ts]; T2 = (R)a.T2s[ts]; }
-                    diff = a.diffusivity[ts];
-                }
-                rec_old = vx.rec;                                  // :165
-                field = fnew;
-            }
-            ph.add(field);                                         // :171
-            itr = 0;
-
-            if (tp == next_ev) {
-                if (tp == next_dp) {                               // :175-178
-                    if (sizeof(R) == 8)
-                        ph.add((R)((double)spin * a.deph_deg[dp_i] / (double)a.n_spins_total));
-                    else
-                        ph.add((R)((float)spin * a.deph_deg[dp_i] / (float)a.n_spins_total));
-                    ++dp_i;
-                    next_dp = (dp_i < a.n_deph) ? (uint32_t)a.deph_t[dp_i] : kNever;
-                }
-                if (tp == next_gr) {                               // :181-187
-                    const float Gx = a.gx[gr_i] * a.gradient_scale;
-                    const float Gy = a.gy[gr_i] * a.gradient_scale;
-                    const float Gz = a.gz[gr_i] * a.gradient_scale;
-                    ph.add_grad((Gx * xn[0] + Gy * xn[1] + Gz * xn[2]) * 1e-3 * a.timestep_us * 1e-6 *
-                                kGamma * kRad2Deg);
-                    ++gr_i;
-                    next_gr = (gr_i < a.n_grad) ? (uint32_t)a.grad_t[gr_i] : kNever;
-                }
-                if (tp == next_rf) {                               // :190-199
-                    const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
-                    dephase_relax(m, ph.angle(), T1, T2, te);
-                    const RfPulse p = a.rf[rf_i];
-                    if (sizeof(R) == 8) rot_apply<R>(p.kind_d, (R)p.sd, (R)p.cd, (R)p.spd, (R)p.cpd, m);
-                    else                rot_apply<R>(p.kind, (R)p.s, (R)p.c, (R)p.sp, (R)p.cp, m);
-                    ph.reset();
-                    old_tp = tp;
-                    ++rf_i;
-                    next_rf = (rf_i < a.n_rf) ? (uint32_t)a.rf[rf_i].t : kNever;
-                }
-                if (tp == next_te) {                               // :202-215
-                    const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
-                    dephase_relax(m, ph.angle(), T1, T2, te);
-                    const uint64_t o = (uint64_t)li * a.n_te + te_i;
-                    R *M1 = (R *)a.M1;
-                    M1[3 * o + 0] = m[0]; M1[3 * o + 1] = m[1]; M1[3 * o + 2] = m[2];
-                    a.T[o] = ts_old;
-                    ph.reset();
-                    old_tp = tp;
-                    ++te_i;
-                    next_te = (te_i < a.n_te) ? (uint32_t)a.TE_t[te_i] : kNever;
-                }
-                next_ev = next_event(next_rf, next_te, next_dp, next_gr);
-            }
-            if (a.record_traj) {                                   // :218-221
-                float *dst = xyz1 + 3 * slot;
-                dst[0] = (float)xn[0]; dst[1] = (float)xn[1]; dst[2] = (float)xn[2];
-                ++slot;
-            }
-            xo[0] = xn[0]; xo[1] = xn[1]; xo[2] = xn[2];
-            ++tp;
-        }
-        const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);   // :226-231
-        dephase_relax(m, ph.angle(), T1, T2, te);
-    }
-    d = a.n_dummy + 1;                                             // every TR done
-    tp = 0;
-done:
-    // Without trajectories XYZ1 holds the last accepted position: the reference
-    // stores every accepted position in place (kernels.cu:222-223), and a spin
-    // that is lost keeps the one before.  Then the count of accepted steps.
-    if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = (float)xo[i];
-    n_acc += (uint64_t)d * a.n_timepoints + tp;
-    return st;
-}
-
-// One workgroup walks the spins at positions [t0, t0 + blockDim.x) of the
-// launch's visiting order (fewer at the end).
-template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
-__device__ __forceinline__ void walk_block(const WalkArgs &a, uint32_t t0) {
-    const uint32_t t = t0 + threadIdx.x;
-    if (IDX32) {
-        const uint32_t n_tab = (uint32_t)(a.nx + a.ny + a.nz);
-        for (uint32_t i = threadIdx.x; i < n_tab; i += blockDim.x) axis_tab[i] = axis_entry<Rec>(a, i);
-        __syncthreads();
-    }
-    uint64_t n_acc = 0;
-    uint32_t n_rej = 0, lost = 0, exh = 0;
-    if (t < a.n_local) {
-        const uint32_t li = a.order ? a.order[t] : t;
-        const int st = walk_spin<Rng, Rec, Phase, CROSS, IDX32>(a, li, n_acc, n_rej);
-        lost = (st == ST_LOST);
-        exh = (st == ST_EXHAUSTED);
-    }
-    const uint64_t s_acc = wave_sum_u64(n_acc), s_rej = wave_sum_u64(n_rej);
-    const uint64_t s_lost = wave_sum_u64(lost), s_exh = wave_sum_u64(exh);
-    if ((threadIdx.x & 63) == 0 && a.counters) {
-        if (s_lost) atomicAdd(&a.counters[0], (unsigned long long)s_lost);
-        if (s_exh)  atomicAdd(&a.counters[1], (unsigned long long)s_exh);
-        atomicAdd(&a.counters[2], (unsigned long long)s_acc);
-        atomicAdd(&a.counters[3], (unsigned long long)(s_acc + s_rej));
-    }
-}
-
-// ---------------------------------------------------------------------------
-// The fused walk: one trajectory, NV views (FusedArgs).  walk_spin_views is
-// walk_spin with the magnetisation, phase, event tables and outputs held per
-// view; the single walk keeps its own function so that its code, which sits at
-// the register limit, is not perturbed.  Batch tests hold every view to the
-// separate walk of its tile, bit for bit.
-// ---------------------------------------------------------------------------
-// View v of a fused walk: view 0 is the WalkArgs fields, views 1.. are xv[v - 1].
-#define SW_VIEW_FIELD(type, name)                                                          \
-    __device__ __forceinline__ type view_##name(const WalkArgs &a, const WalkView *xv, int v) { \
-        return v == 0 ? a.name : xv[v - 1].name;                                           \
-    }
-SW_VIEW_FIELD(const RfPulse *, rf)
-SW_VIEW_FIELD(const Excitation *, exc)
-SW_VIEW_FIELD(const float *, deph_deg)
-SW_VIEW_FIELD(const float *, gx)
-SW_VIEW_FIELD(const float *, gy)
-SW_VIEW_FIELD(const float *, gz)
-SW_VIEW_FIELD(float, exc_s)
-SW_VIEW_FIELD(float, exc_c)
-SW_VIEW_FIELD(double, exc_sd)
-SW_VIEW_FIELD(double, exc_cd)
-SW_VIEW_FIELD(float, gradient_scale)
-SW_VIEW_FIELD(void *, M1)
-SW_VIEW_FIELD(float *, XYZ1)
-SW_VIEW_FIELD(uint8_t *, T)
-SW_VIEW_FIELD(unsigned long long *, counters)
-#undef SW_VIEW_FIELD
-
-// One spin.  NV views share the trajectory (draws, moves, lookups, gathers,
-// labels and their tissue parameters); each view keeps its own magnetisation
-// and phase, formed by the operations of its own walk (walk_spin above).
-template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32, int NV>
-__device__ int walk_spin_views(const WalkArgs &a, const WalkView *xv, uint32_t li, uint64_t &n_acc,
-                               uint32_t &n_rej) {
-    using R = typename Phase::R;
-    const uint32_t spin = a.spin_offset + li;
-    const uint64_t xoff = 3 * (uint64_t)li * a.traj_len;          // this spin's XYZ1 slots
-    float *const xyz1_0 = a.XYZ1 + xoff;                           // view 0's, kept in registers
-    auto xyz1_of = [&](int v) -> float * { return v == 0 ? xyz1_0 : view_XYZ1(a, xv, v) + xoff; };
-    double xo[3], xn[3];
-    R m[NV][3];
-    for (int i = 0; i < 3; i++) {                                  // kernels.cu:95-99
-        const float x0 = a.XYZ0[3 * (uint64_t)li + i] * a.xyz_scale;
-        xo[i] = xn[i] = (double)x0;
-        for (int v = 0; v < NV; v++) m[v][i] = (R)a.M0[3 * (uint64_t)li + i];
-        if (a.record_traj)
-            for (int v = 0; v < NV; v++) xyz1_of(v)[i] = x0;
-    }
-    Vox<IDX32> vx = locate<Rec, IDX32>(a, xo);                     // kernels.cu:102
-    if (!vx.ok) {                                                  // the reference reads out of bounds here
-        if (!a.record_traj)
-            for (int v = 0; v < NV; v++)
-                for (int i = 0; i < 3; i++) xyz1_of(v)[i] = (float)xo[i];
-        return ST_LOST;
-    }
-    uint8_t ts_old = Rec::label(load_rec<typename Rec::T, IDX32>(a.rec, vx.rec));   // :103
-    double diff = a.diffusivity[ts_old];                           // :104
-    // The reference re-reads T1/T2/diffusivity of ts_old at every voxel change
-    // (:167-169); they only change with ts_old, so they are read when it changes.
-    // Before the first accepted step T1 = T2 = 0 (:91); no event can use them
-    // before that step unless the TR has no time points at all.
-    R field = 0, T1 = 0, T2 = 0;
-    if (a.n_timepoints > 0) {
-        if (sizeof(R) == 8) { T1 = (R)a.T1d[ts_old]; T2 = (R)a.T2d[ts_old]; }
-        else                { T1 = (R)a.T1s[ts_old]; T2 = (R)a.T2s[ts_old]; }
-    }
-    uint32_t itr = 0;
-    Rng rng(a, li, spin);
-    rng.peek();
-    Phase ph[NV];
-    uint64_t slot = 0;
-    // Every early end (lost, stuck or out of draws) leaves through `done`, so the
-    // step loop has one exit edge besides its natural end.
-    int st = ST_OK;
-    uint32_t d = 0, tp = 0;
-
-    for (d = 0; d <= a.n_dummy; ++d) {                              // :107
-        const bool last = (d == a.n_dummy);
-        for (int v = 0; v < NV; v++) {                             // :110-120 excitation
-            const Excitation e = view_exc(a, xv, v)[d];
-            if (sizeof(R) == 8)
-                rot_apply<R>(e.kind_d, (R)view_exc_sd(a, xv, v), (R)view_exc_cd(a, xv, v), (R)e.spd, (R)e.cpd, m[v]);
-            else
-                rot_apply<R>(e.kind, (R)view_exc_s(a, xv, v), (R)view_exc_c(a, xv, v), (R)e.sp, (R)e.cp, m[v]);
-        }
-        RecIdx<IDX32> rec_old = ~(RecIdx<IDX32>)0;                 // :123-126 (ind_old: no voxel)
-        uint32_t old_tp = 0;
-        tp = 0;
-        uint32_t rf_i = 1, te_i = 0, dp_i = 0, gr_i = 0;
-        uint32_t next_rf = (a.n_rf > 1) ? (uint32_t)a.rf[1].t : kNever;
-        uint32_t next_te = (last && a.n_te > 0) ? (uint32_t)a.TE_t[0] : kNever;
-        uint32_t next_dp = (a.n_deph > 0) ? (uint32_t)a.deph_t[0] : kNever;
-        uint32_t next_gr = (a.n_grad > 0) ? (uint32_t)a.grad_t[0] : kNever;
-        uint32_t next_ev = next_event(next_rf, next_te, next_dp, next_gr);
-        for (int v = 0; v < NV; v++) ph[v].reset();
-
-        while (tp < a.n_timepoints) {                              // :128
-            const bool moved = diff != 0.;
-            if (moved) {                                           // :130-137
-                if (!rng.avail) { st = ST_EXHAUSTED; goto done; }
-                move<CROSS>(a, rng.px, rng.py, rng.pz, diff, xo, xn);
-            }
-            vx = locate<Rec, IDX32>(a, xn);                        // :140
-            if (!vx.ok) { st = ST_LOST; goto done; }               // :141-147
-            const bool newvox = vx.rec != rec_old;                 // :150
-            typename Rec::T r{};
-            if (newvox)                                            // one gather: label and field
-                r = load_rec<typename Rec::T, IDX32>(a.rec, vx.rec);
-            if (moved) {                                           // next attempt's draws while the load flies
-                rng.commit();
-                rng.peek();
-            }
-            if (newvox) {
-                const uint8_t ts = Rec::label(r);                  // :152
-                const R fnew = Rec::template field<R>(a, r);       // :166, read speculatively
-                if (ts != ts_old) {                                // :153-163
-                    float u;
-                    if (!rng.uniform(u)) { st = ST_EXHAUSTED; goto done; }
-                    if (u >= a.pXY[ts_old * a.n_sub + ts]) {
-                        ++n_rej;
-                        if (itr++ > a.max_iterations) { st = ST_LOST; goto done; }
-                        continue;
-                    }
-                    ts_old = ts;
-                    if (sizeof(R) == 8) { T1 = (R)a.T1d[ts]; T2 = (R)a.T2d[ts]; }
-                    else                { T1 = (R)a.T1s[ts]; T2 = (R)a.T2s[ts]; }
-                    diff = a.diffusivity[ts];
-                }
-                rec_old = vx.rec;                                  // :165
-                field = fnew;
-            }
-            for (int v = 0; v < NV; v++) ph[v].add(field);         // :171
-            itr = 0;
-
-            if (tp == next_ev) {
-                if (tp == next_dp) {                               // :175-178
-                    for (int v = 0; v < NV; v++) {
-                        const float dd = view_deph_deg(a, xv, v)[dp_i];
-                        if (sizeof(R) == 8)
-                            ph[v].add((R)((double)spin * dd / (double)a.n_spins_total));
-                        else
-                            ph[v].add((R)((float)spin * dd / (float)a.n_spins_total));
-                    }
-                    ++dp_i;
-                    next_dp = (dp_i < a.n_deph) ? (uint32_t)a.deph_t[dp_i] : kNever;
-                }
-                if (tp == next_gr) {                               // :181-187
-                    for (int v = 0; v < NV; v++) {
-                        const float gs = view_gradient_scale(a, xv, v);
-                        const float Gx = view_gx(a, xv, v)[gr_i] * gs;
-                        const float Gy = view_gy(a, xv, v)[gr_i] * gs;
-                        const float Gz = view_gz(a, xv, v)[gr_i] * gs;
-                        ph[v].add_grad((Gx * xn[0] + Gy * xn[1] + Gz * xn[2]) * 1e-3 * a.timestep_us * 1e-6 *
-                                       kGamma * kRad2Deg);
-                    }
-                    ++gr_i;
-                    next_gr = (gr_i < a.n_grad) ? (uint32_t)a.grad_t[gr_i] : kNever;
-                }
-                if (tp == next_rf) {                               // :190-199
-                    const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
-                    for (int v = 0; v < NV; v++) {
-                        dephase_relax(m[v], ph[v].angle(), T1, T2, te);
-                        const RfPulse p = view_rf(a, xv, v)[rf_i];
-                        if (sizeof(R) == 8) rot_apply<R>(p.kind_d, (R)p.sd, (R)p.cd, (R)p.spd, (R)p.cpd, m[v]);
-                        else                rot_apply<R>(p.kind, (R)p.s, (R)p.c, (R)p.sp, (R)p.cp, m[v]);
-                        ph[v].reset();
-                    }
-                    old_tp = tp;
-                    ++rf_i;
-                    next_rf = (rf_i < a.n_rf) ? (uint32_t)a.rf[rf_i].t : kNever;
-                }
-                if (tp == next_te) {                               // :202-215
-                    const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
-                    const uint64_t o = (uint64_t)li * a.n_te + te_i;
-                    for (int v = 0; v < NV; v++) {
-                        dephase_relax(m[v], ph[v].angle(), T1, T2, te);
-                        R *M1 = (R *)view_M1(a, xv, v);
-                        M1[3 * o + 0] = m[v][0]; M1[3 * o + 1] = m[v][1]; M1[3 * o + 2] = m[v][2];
-                        view_T(a, xv, v)[o] = ts_old;
-                        ph[v].reset();
-                    }
-                    old_tp = tp;
-                    ++te_i;
-                    next_te = (te_i < a.n_te) ? (uint32_t)a.TE_t[te_i] : kNever;
-                }
-                next_ev = next_event(next_rf, next_te, next_dp, next_gr);
-            }
-            if (a.record_traj) {                                   // :218-221
-                for (int v = 0; v < NV; v++) {
-                    float *dst = xyz1_of(v) + 3 * slot;
-                    dst[0] = (float)xn[0]; dst[1] = (float)xn[1]; dst[2] = (float)xn[2];
-                }
-                ++slot;
-            }
-            xo[0] = xn[0]; xo[1] = xn[1]; xo[2] = xn[2];
-            ++tp;
-        }
-        const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);   // :226-231
-        for (int v = 0; v < NV; v++) dephase_relax(m[v], ph[v].angle(), T1, T2, te);
-    }
-    d = a.n_dummy + 1;                                             // every TR done
-    tp = 0;
-done:
-    // Without trajectories XYZ1 holds the last accepted position: the reference
-    // stores every accepted position in place (kernels.cu:222-223), and a spin
-    // that is lost keeps the one before.  Then the count of accepted steps.
-    if (!a.record_traj)
-        for (int v = 0; v < NV; v++)
-            for (int i = 0; i < 3; i++) xyz1_of(v)[i] = (float)xo[i];
-    n_acc += (uint64_t)d * a.n_timepoints + tp;
-    return st;
-}
-
-
-// One workgroup walks the spins at positions [t0, t0 + blockDim.x) of the
-// launch's visiting order (fewer at the end).
-template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32, int NV>
-__device__ __forceinline__ void walk_block_views(const WalkArgs &a, const WalkView *xv, uint32_t n_views,
-                                                 uint32_t t0) {
-    const uint32_t t = t0 + threadIdx.x;
-    if (IDX32) {
-        const uint32_t n_tab = (uint32_t)(a.nx + a.ny + a.nz);
-        for (uint32_t i = threadIdx.x; i < n_tab; i += blockDim.x) axis_tab[i] = axis_entry<Rec>(a, i);
-        __syncthreads();
-    }
-    uint64_t n_acc = 0;
-    uint32_t n_rej = 0, lost = 0, exh = 0;
-    if (t < a.n_local) {
-        const uint32_t li = a.order ? a.order[t] : t;
-        const int st = walk_spin_views<Rng, Rec, Phase, CROSS, IDX32, NV>(a, xv, li, n_acc, n_rej);
-        lost = (st == ST_LOST);
-        exh = (st == ST_EXHAUSTED);
-    }
-    const uint64_t s_acc = wave_sum_u64(n_acc), s_rej = wave_sum_u64(n_rej);
-    const uint64_t s_lost = wave_sum_u64(lost), s_exh = wave_sum_u64(exh);
-    if ((threadIdx.x & 63) == 0) {
-        for (int v = 0; v < NV; v++) {                 // every view walked these steps
-            unsigned long long *c = view_counters(a, xv, v);
-            if (c && (v == 0 || (uint32_t)v < n_views)) {
-                if (s_lost) atomicAdd(&c[0], (unsigned long long)s_lost);
-                if (s_exh)  atomicAdd(&c[1], (unsigned long long)s_exh);
-                atomicAdd(&c[2], (unsigned long long)s_acc);
-                atomicAdd(&c[3], (unsigned long long)(s_acc + s_rej));
-            }
-        }
-    }
-}
-
-// Occupancy target: waves per SIMD the register allocation must allow.
-#ifndef SW_WALK_WAVES_PER_EU
-#define SW_WALK_WAVES_PER_EU 6
-#endif
-
-template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SW_WALK_WAVES_PER_EU)))
-void walk_kernel(const WalkArgs a) {
-    // Blocks are dealt round-robin over the 8 XCDs; remap so that each XCD gets a
-    // contiguous range of the visiting order (spatially sorted spins share L2).
-    // The grid is a multiple of 8 blocks, so the remap is a permutation.
-    const uint32_t per = gridDim.x / 8, b = blockIdx.x;
-    const uint32_t lb = (b % 8) * per + b / 8;
-    walk_block<Rng, Rec, Phase, CROSS, IDX32>(a, lb * blockDim.x);
-}
-
-// The batched walk: block b takes the chunk at slot (b mod 8) * (grid / 8) + b / 8
-// of the chunk table, i.e. XCD k works through slots [k * grid/8, (k+1) * grid/8)
-// in order.  The host lays each XCD's slots out longest tile first, so the long
-// walks start early and the short ones fill in at the end.  NV > 1: fused items
-// of up to NV views, whose per-view state costs registers, so the occupancy
-// target is one wave lower.
-// Occupancy of the fused kernels: the per-view state (magnetisation and one
-// phase register per view, two with Kahan compensation) leaves room for 5 waves
-// per SIMD, FP32 and Kahan phases alike, with no spill in the step loop; the few
-// spills at 5 waves sit in the TR set-up and the trajectory stores.  Measured:
-// 64-phantom FP32 batch 69.6 ms at 5, 74.1 ms at 4; Kahan + int16 field 68.5 ms
-// at 5, 75.0 ms at 4; 1000^3 bSSFP Kahan + int16 field 486.5 ms at 5, 541.6 ms
-// at 4.  (With the exact remainder by 360 written as a loop, the Kahan kernel
-// spilled in the step loop at 5 and ran slower there.)
-template <class Phase, int NV>
-constexpr int walk_waves_per_eu() {
-    return NV == 1 ? SW_WALK_WAVES_PER_EU : 5;
-}
-
-template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32, int NV>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(walk_waves_per_eu<Phase, NV>())))
-void walk_batch_kernel(const void *__restrict__ items, const BatchChunk *__restrict__ chunks) {
-    const uint32_t per = gridDim.x / 8, b = blockIdx.x;
-    const BatchChunk c = chunks[(b % 8) * per + b / 8];
-    if (c.item == kNoItem) return;                    // padding slot (uniform across the block)
-    if constexpr (NV == 1) {
-        walk_block<Rng, Rec, Phase, CROSS, IDX32>(((const WalkArgs *)items)[c.item], c.first);
-    } else {
-        const FusedArgs &f = ((const FusedArgs *)items)[c.item];
-        walk_block_views<Rng, Rec, Phase, CROSS, IDX32, NV>(f.a, f.xv, f.n_views, c.first);
-    }
-}
-
-// 32-bit mode: record byte offsets below 2^32 and the per-axis tables within
-// 64 KiB of LDS.
-template <class F>
-WalkMode mode_of(const WalkArgs &a) {
-    constexpr int lb = F::shape.lbx + F::shape.lby + F::shape.lbz;
-    const uint64_t records = brick_count(a.nx, F::shape.lbx) * a.nb_y * a.nb_z << lb;
-    const uint64_t n_tab = (uint64_t)a.nx + (uint64_t)a.ny + (uint64_t)a.nz;
-    WalkMode m;
-    m.idx32 = records * sizeof(typename F::T) < (1ull << 32) && n_tab * 4 <= 65536;
-    m.lds = m.idx32 ? (size_t)n_tab * 4 : 0;
-    return m;
-}
-
-template <class Rng, class F, class P>
-hipError_t launch_fp(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
-    const WalkMode m = mode_of<F>(a);
-    const bool idx32 = m.idx32;
-    const size_t lds = m.lds;
-    if (a.cross_fov) {
-        if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, true>), grid, block, lds, s, a);
-        else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, false>), grid, block, 0, s, a);
-    } else {
-        if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, false, true>), grid, block, lds, s, a);
-        else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, false, false>), grid, block, 0, s, a);
-    }
-    return hipGetLastError();
-}
-
-template <class Rng, class F, class P, int NV>
-hipError_t launch_batch_nv(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                           bool cross, bool idx32, size_t lds, hipStream_t s) {
-    const dim3 grid(n_slots), block(threads);
-    if (cross) {
-        if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, true, NV>), grid, block, lds, s, items, chunks);
-        else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, false, NV>), grid, block, 0, s, items, chunks);
-    } else {
-        if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, false, true, NV>), grid, block, lds, s, items, chunks);
-        else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, false, false, NV>), grid, block, 0, s, items, chunks);
-    }
-    return hipGetLastError();
-}
-
-// views: 1, or kMaxViews for fused items (FP32 and Kahan phases only).
-template <class Rng, class F, class P>
-hipError_t launch_batch_fp(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                           int views, bool cross, bool idx32, size_t lds, hipStream_t s) {
-    if (views == 1) return launch_batch_nv<Rng, F, P, 1>(items, chunks, n_slots, threads, cross, idx32, lds, s);
-    if constexpr (sizeof(typename P::R) == 4) {
-        if (views == kMaxViews)
-            return launch_batch_nv<Rng, F, P, kMaxViews>(items, chunks, n_slots, threads, cross, idx32, lds, s);
-    }
-    return hipErrorInvalidValue;
-}
-
-template <class Rng>
-hipError_t launch_batch_rng(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                            int views, int field, int prec, bool cross, bool idx32, size_t lds, hipStream_t s) {
-    if (prec == 2) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF64>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF64>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-    } else if (prec == 1) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-    } else if (prec == 0) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-    }
-    return hipErrorInvalidValue;
-}
-
-template <class Rng>
-hipError_t launch_rng(const WalkArgs &a, int field, int prec, hipStream_t s) {
-    const uint32_t nb = (a.n_local + 255) / 256;
-    if (nb == 0) return hipSuccess;
-    const dim3 block(256), grid((nb + 7) / 8 * 8);
-    if (prec == 2) {
-        if (field == 0) return launch_fp<Rng, RecNone, PhaseF64>(a, s, grid, block);
-        if (field == 1) return launch_fp<Rng, RecF32, PhaseF64>(a, s, grid, block);
-    } else if (prec == 1) {
-        if (field == 0) return launch_fp<Rng, RecNone, PhaseKahan>(a, s, grid, block);
-        if (field == 1) return launch_fp<Rng, RecF32, PhaseKahan>(a, s, grid, block);
-        if (field == 2) return launch_fp<Rng, RecF16, PhaseKahan>(a, s, grid, block);
-        if (field == 3) return launch_fp<Rng, RecBF16, PhaseKahan>(a, s, grid, block);
-        if (field == 4) return launch_fp<Rng, RecQ16, PhaseKahan>(a, s, grid, block);
-    } else if (prec == 0) {
-        if (field == 0) return launch_fp<Rng, RecNone, PhaseF32>(a, s, grid, block);
-        if (field == 1) return launch_fp<Rng, RecF32, PhaseF32>(a, s, grid, block);
-        if (field == 2) return launch_fp<Rng, RecF16, PhaseF32>(a, s, grid, block);
-        if (field == 3) return launch_fp<Rng, RecBF16, PhaseF32>(a, s, grid, block);
-        if (field == 4) return launch_fp<Rng, RecQ16, PhaseF32>(a, s, grid, block);
-    }
-    return hipErrorInvalidValue;
-}
-
-}  // namespace
 
 hipError_t launch_walk(const WalkArgs &a, int rng, int field, int prec, hipStream_t s) {
-    if (rng == 1) return launch_rng<ReplayRng>(a, field, prec, s);
-    if (rng == 0) return launch_rng<PhiloxRng>(a, field, prec, s);
+    if (rng == 1) return launch_walk_replay(a, field, prec, s);
+    if (rng == 0) return launch_walk_philox(a, field, prec, s);
     return hipErrorInvalidValue;
 }
 
@@ -988,10 +26,8 @@ hipError_t launch_walk_batch(const void *items, const BatchChunk *chunks, uint32
                              hipStream_t s) {
     if (n_slots == 0) return hipSuccess;
     if (n_slots % 8 || threads == 0 || threads > 256 || threads % 64) return hipErrorInvalidValue;
-    if (rng == 1)
-        return launch_batch_rng<ReplayRng>(items, chunks, n_slots, threads, views, field, prec, cross, idx32, lds, s);
-    if (rng == 0)
-        return launch_batch_rng<PhiloxRng>(items, chunks, n_slots, threads, views, field, prec, cross, idx32, lds, s);
+    if (rng == 1) return launch_walk_batch_replay(items, chunks, n_slots, threads, views, field, prec, cross, idx32, lds, s);
+    if (rng == 0) return launch_walk_batch_philox(items, chunks, n_slots, threads, views, field, prec, cross, idx32, lds, s);
     return hipErrorInvalidValue;
 }
 
@@ -1020,7 +56,7 @@ __global__ void philox_draws_kernel(uint64_t seed, uint32_t spin0, uint32_t n_sp
     PhiloxRng r(a, 0, spin0 + i);
     r.attempt = k;
     r.peek();
-    out[4 * g + 0] = r.px; out[4 * g + 1] = r.py; out[4 * g + 2] = r.pz; out[4 * g + 3] = r.pu;
+    out[4 * g + 0] = r.px; out[4 * g + 1] = r.py; out[4 * g + 2] = r.pz; out[4 * g + 3] = u24_to_unit(r.pu);
 }
 
 hipError_t launch_philox_draws(uint64_t seed, uint32_t spin0, uint32_t n_spins, uint32_t n_att,

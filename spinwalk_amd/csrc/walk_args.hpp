@@ -81,6 +81,12 @@ struct WalkArgs {
     double          nd[3];        // nx, ny, nz as doubles
     double          fov[3];
     double          s2g[3];       // phantom_size / fov (kernels.cu:98)
+    // the production walk's voxel-unit positions (PosVox)
+    float           s2gf[3];      // (float) s2g: metres -> voxels
+    float           vsf[3];       // (float) fov / n: voxels -> metres
+    float           nf[3];        // nx, ny, nz as floats: the FoV in voxels
+    double          vs[3];        // fov / n in double (the FP64 walk's gradient phase)
+    float           g2d;          // (float) 1e-3 dt 1e-6 gamma rad2deg: mT/m x m -> degrees
 
     // ---- sequence tables (device) -------------------------------------------
     const double   *diffusivity;  // [n_sub] metres per axis per step

@@ -58,22 +58,21 @@ def signal(M1, T, te, label=None):
 
 
 # ---------------------------------------------------------------------------
-def test_philox_kat():
-    """Random123 known answers for Philox4x32-10 (the host build of philox.hpp)."""
-    import ctypes, os, subprocess, tempfile
-    src = os.path.join(os.path.dirname(__file__), "..", "spinwalk_amd", "csrc", "philox.hpp")
-    code = ('#include "%s"\n#include <cstdio>\nint main(){auto a=sw::philox4x32_10(0,0,0,0,0,0);'
-            'auto b=sw::philox4x32_10(~0u,~0u,~0u,~0u,~0u,~0u);'
-            'auto c=sw::philox4x32_10(0x243f6a88,0x85a308d3,0x13198a2e,0x03707344,0xa4093822,0x299f31d0);'
-            'printf("%%08x%%08x%%08x%%08x %%08x%%08x%%08x%%08x %%08x%%08x%%08x%%08x",a.v[0],a.v[1],a.v[2],a.v[3],'
-            'b.v[0],b.v[1],b.v[2],b.v[3],c.v[0],c.v[1],c.v[2],c.v[3]);}') % os.path.abspath(src)
-    d = tempfile.mkdtemp()
-    with open(os.path.join(d, "k.cpp"), "w") as fh:
-        fh.write(code)
-    subprocess.run(["g++", "-O2", "-o", os.path.join(d, "k"), os.path.join(d, "k.cpp")], check=True)
-    out = subprocess.run([os.path.join(d, "k")], capture_output=True, text=True, check=True).stdout.split()
-    assert out == ["6627e8d5e169c58dbc57ac4c9b00dbd8", "408f276d41c83b0ea20bc7c66d5451fd",
-                   "d16cfe0994fdcceb5001e42024126ea1"]
+def test_device_draws_equal_the_restatement(engine):
+    """The device's draws (sw_philox_draws_device: the bundle each attempted step
+    takes) against the numpy restatement of Philox4x32-10 and of the bundle, with
+    a 64-bit seed and a spin offset: the permeability uniform bit for bit, the
+    normals within the error of the hardware log2, sqrt, sin and cos."""
+    import torch
+    from tests.test_philox import draw_bundle
+    seed, spin0, n_spins, n_att = 0x0123456789ABCDEF, 1000, 512, 64
+    out = torch.zeros((n_spins, n_att, 4), dtype=torch.float32, device="cuda")
+    engine.philox_draws(seed, spin0, n_spins, n_att, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    want = draw_bundle(seed, spin0, n_spins, n_att)
+    np.testing.assert_array_equal(got[..., 3], want[..., 3].astype(np.float32))
+    np.testing.assert_allclose(got[..., :3], want[..., :3], rtol=0, atol=2e-5)
 
 
 def test_device_draw_statistics(engine):
@@ -153,10 +152,12 @@ def test_free_diffusion_increments(engine):
 
 
 def test_walk_consumes_the_materialized_draws(engine):
-    """Every attempted step k of spin s moves by (double)draw[s, k, 0:3] * diff, with
-    draw = sw_philox_draws_device(seed, s, k): replaying the move arithmetic of
-    kernels.cu:131 in float64 on the recorded draws gives the recorded trajectory
-    bit for bit.  One substrate, no walls within reach: every attempt is accepted."""
+    """Every attempted step k of spin s moves by draw[s, k, 0:3] times the step
+    size sqrt(2 D dt) (simulation_parameters.cuh:236-237, kernels.cu:131), with
+    draw = sw_philox_draws_device(seed, s, k): each increment of the recorded
+    trajectory equals it to the float rounding of the positions (the production
+    walk keeps positions in float voxel units).  One substrate, no walls within
+    reach: every attempt is accepted."""
     import torch
     mask = np.zeros((8, 8, 8), np.uint8)
     cfg = cases.gre(1, TE=5000, dt=50, DIFFUSIVITY=[2e-9], RECORD_TRAJECTORY=1, T1=[1e9], T2=[1e9])
@@ -167,17 +168,48 @@ def test_walk_consumes_the_materialized_draws(engine):
     g = run(engine, case)
     n_t = case.traj_len
     assert g["stats"]["n_lost"] == 0 and n_t > 50
+    assert g["stats"]["n_attempts"] == g["stats"]["n_steps"] == n * n_t
     d = torch.zeros((n, n_t, 4), dtype=torch.float32, device="cuda")
     engine.philox_draws(case.params.seed, 0, n, n_t, d)
     torch.cuda.synchronize()
-    draws = d.cpu().numpy()
+    draws = d.cpu().numpy().astype(np.float64)
     diff = float(case.tables["diffusivity"][0])
-    x = xyz0.astype(np.float64)
-    want = np.empty((n, n_t, 3), np.float32)
-    for k in range(n_t):
-        x = x + draws[:, k, :3].astype(np.float64) * diff
-        want[:, k] = x.astype(np.float32)
-    np.testing.assert_array_equal(g["XYZ1"], want)
+    X = g["XYZ1"].astype(np.float64)
+    prev = np.concatenate([xyz0.astype(np.float64)[:, None, :], X[:, :-1]], axis=1)
+    err = np.abs((X - prev) - draws[..., :3] * diff)
+    ulp = float(np.spacing(np.float32(600e-6)))                  # one float step of a position near 0.6 mm
+    assert err.max() < 4 * ulp, (err.max(), ulp, diff)
+    assert diff > 1e3 * ulp                                      # the check resolves a step to 1e-3 of its size
+
+
+def test_production_boundaries_match_the_reference(engine):
+    """The FoV boundary in the production walk (float voxel units) against the
+    oracle's (the reference CPU path, kernels.cu:133-136): spins that start
+    uniform over a small FoV and cross its faces many times stay inside it with
+    none lost, and their final positions follow the oracle's distribution
+    (two-sample KS per axis).  The reference's CROSS_FOV = 0 correction turns a
+    crossing step back from the spin's old position (x_old + |step|), which is
+    not a mirror image and thins the spins near the walls; wrap-around
+    (CROSS_FOV = 1) keeps them uniform."""
+    from scipy import stats
+    mask = np.zeros((8, 8, 8), np.uint8)
+    for cross in (0, 1):
+        cfg = cases.gre(1, TE=20000, dt=50, DIFFUSIVITY=[3e-9], CROSS_FOV=cross, T1=[1e9], T2=[1e9])
+        case = cases.make_case(cfg, mask, None, 8.0, n_spins=20000)      # steps of 0.55 um in an 8 um box
+        g = run(engine, case)
+        o = orc.sim(cases.to_oracle(case))
+        assert g["stats"]["n_lost"] == 0 and o["n_lost"] == 0
+        fov = float(case.fov[0])
+        x = g["XYZ1"][:, 0, :].astype(np.float64) / fov
+        xo = o["XYZ1"][:, 0, :].astype(np.float64) / fov
+        assert (x >= 0).all() and (x <= 1).all()
+        for k in range(3):
+            assert stats.ks_2samp(x[:, k], xo[:, k]).pvalue > 1e-4, (cross, k)
+            if cross:
+                assert stats.kstest(x[:, k], "uniform").pvalue > 1e-4, k
+        if not cross:                                   # the thinning near the walls is there in both
+            edge = lambda v: float(((v < 0.03) | (v > 0.97)).mean())
+            assert edge(x[:, 0]) < 0.05 and edge(xo[:, 0]) < 0.05
 
 
 def test_pgse_discrete_stejskal_tanner(engine):
