@@ -12,7 +12,11 @@ engine at the reference's precision (FP32), all tiles in one launch of one
 captured HIP graph.  Beside it, the same tiles walked as the reference launches
 them (one plain FP32 walk per tile in turn) and the batched engine in mixed
 precision (Kahan-compensated FP32 phase, int16 fixed-point field), with every
-signal's error against the FP64 walk of the same draws.
+signal's error against the FP64 walk of the same draws.  Before the timed
+region each batch runs the engine's autotuner (sw_batch_tune: chunk size,
+bands, spins per lane, resident waves per SIMD, visiting-order brick) and
+keeps the schedule it picks; the line reports the schedule and the autotuner's
+times (`--no-tune` keeps the default schedule, `--schedule` sets one).
 
 ``--workload c1|c2|c3|c5`` selects another workload (c1: 100^3 GRE, 1e5 spins;
 c2: 500^3 spin echo, 1e6 spins, one walk; c3: 1000^3 bSSFP, four phase-cycling
@@ -198,13 +202,16 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
             items.append(it)
         return eng.batch(items), items
 
-    def run_batch(precision, field, label):
+    def run_batch(precision, field, label, schedule):
         b, items = make_batch(precision, field)
         tuning = None
-        if args.tune:
+        if schedule:                       # a given schedule (a profiled run repeats a tuned one)
+            b.set_schedule(*[int(x) for x in schedule.split(",")])
+            log(f"[bench] schedule ({label}): {b.schedule}")
+        elif not args.no_tune:             # the autotuner, before the timed region
             tuning = b.tune(stream=stream)
             log(f"[bench] autotuner ({label}): {tuning['schedule']}, {tuning['best_ms']:.2f} ms per run "
-                f"(starting schedule {tuning['first_ms']:.2f} ms)")
+                f"(default schedule {tuning['first_ms']:.2f} ms)")
         st = {}
 
         def batched():
@@ -220,8 +227,8 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
 
     seq_s = timed(sequential)
     log(f"[bench] one plain FP32 walk per tile: {seq_s * 1e3:.2f} ms, {total_steps / seq_s:.4g} spin-steps/s")
-    f32 = run_batch(0, E.SW_FIELD_F32, "FP32")
-    mixed = run_batch(prec, fmt, f"{args.precision or 'kahan'} + {args.field or 'q16'} field")
+    f32 = run_batch(0, E.SW_FIELD_F32, "FP32", args.schedule)
+    mixed = run_batch(prec, fmt, f"{args.precision or 'kahan'} + {args.field or 'q16'} field", args.schedule_mixed)
     for r in (f32, mixed):
         r["batch"].close()
         del r["items"]
@@ -259,13 +266,18 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
         bpe = 5 if field == E.SW_FIELD_F32 else 3           # u8 mask + field bytes per spin-step
         kernel_s = res["stats"]["kernel_ms"] * 1e-3
         achieved = total_steps * bpe / kernel_s / 1e9
-        traffic = None
+        traffic, traffic_note = None, "no rocprof measurement for this workload and form in profiles/"
         tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{tag}.json")
         if os.path.exists(tpath):
             with open(tpath) as fh:
-                traffic = json.load(fh).get("hbm_bytes_per_launch")
+                t = json.load(fh)
+            # the line traffic depends on the schedule: report it only for the schedule it was measured under
+            if t.get("schedule") == res["schedule"]:
+                traffic, traffic_note = t.get("hbm_bytes_per_launch"), f"measured, {os.path.relpath(tpath, ROOT)}"
+            else:
+                traffic_note = f"{os.path.relpath(tpath, ROOT)} was measured under schedule {t.get('schedule')}"
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
                 "traffic_rate": None if traffic is None else traffic / kernel_s / 1e9,
                 "bytes_per_spin_step": bpe, "kernel_ms": res["stats"]["kernel_ms"], "kernel": "walk_batch_kernel"}
 
@@ -370,8 +382,13 @@ def main():
                     help="after the timed region, time one host-buffer call (sw_walk_host: uploads, walk, downloads)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed region, run one device copy of this size (profiler calibration)")
-    ap.add_argument("--tune", action="store_true",
-                    help="tile workloads: run the batch autotuner (sw_batch_tune) before the timed region")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="tile workloads: keep the default schedule instead of running the batch autotuner "
+                         "(sw_batch_tune) before the timed region")
+    ap.add_argument("--schedule", default=None,
+                    help="tile workloads: the FP32 batch's schedule, 'chunk,bands_shared,bands_mixed,"
+                         "spins_per_lane,waves_per_simd,order_brick' (no autotuning)")
+    ap.add_argument("--schedule-mixed", default=None, help="the same for the mixed-precision batch")
     ap.add_argument("--probe", action="store_true",
                     help="instead of the walk: the memory ceilings of the gather (random 128-byte line reads at "
                          "several working-set sizes) and of a streaming copy")

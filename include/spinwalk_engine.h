@@ -291,22 +291,30 @@ SW_API int  sw_batch_graph_info(const sw_batch *b, int32_t *uses_graph, uint32_t
 SW_API void sw_batch_destroy(sw_batch *b);
 
 /* ---- batch schedule and its autotuner (no counterpart in the reference) -----
- * The order in which a batch's 256-spin chunks are walked.  Every tile is split
- * into 8 parts, one per XCD; within an XCD, tiles of one cost class (a factor
- * of two in steps per spin) take turns in bands through their parts, longer
- * classes first.  A schedule changes only that order: results are the same bit
- * for bit under every schedule. */
+ * How a batch is walked.  Its spins go in chunks of chunk_spins * spins_per_lane
+ * (one workgroup of chunk_spins lanes each).  Every tile is split into 8 parts,
+ * one per XCD; within an XCD, tiles of one cost class (a factor of two in steps
+ * per spin) take turns in bands through their parts, longer classes first.
+ * waves_per_simd caps the waves resident on a SIMD (through the workgroups' LDS
+ * allocation), which sets how many spins share each XCD's L2; order_brick
+ * is the edge of the voxel bricks whose Morton order is each tile's visiting
+ * order.  A schedule changes only which spins run where and when: results are
+ * the same bit for bit under every schedule. */
 typedef struct sw_batch_schedule {
-    uint32_t chunk_spins;    /* spins per chunk = workgroup size: 64, 128 or 256        */
+    uint32_t chunk_spins;    /* lanes per workgroup: 64, 128 or 256                    */
     uint32_t bands_shared;   /* bands per part when every tile of a class walks one phantom */
     uint32_t bands_mixed;    /* bands per part for the other classes                   */
+    uint32_t spins_per_lane; /* 1 or 2: spins a lane walks, one after the other        */
+    uint32_t waves_per_simd; /* 0: as many as registers allow; 2-8: at most that many   */
+    uint32_t order_brick;    /* 4, 8, 16 or 32 voxels                                  */
 } sw_batch_schedule;
 SW_API int  sw_batch_get_schedule(const sw_batch *b, sw_batch_schedule *out);
 /* Rebuilds the chunk table and the graph for `s` (a tuned schedule, say). */
 SW_API int  sw_batch_set_schedule(sw_batch *b, const sw_batch_schedule *s);
 /* Times full runs of the batch on `stream` under candidate schedules (chunk size,
- * then the band counts of the class kinds present), keeps the fastest and
- * recaptures the graph.  At most max_trials timed runs after two of the current
+ * the band counts of the class kinds present, spins per lane, resident waves
+ * per SIMD, visiting-order brick), keeps the fastest and recaptures
+ * the graph.  At most max_trials timed runs after two of the current
  * schedule (0: no limit).  *best_ms / *first_ms: the kept schedule's run and the
  * starting schedule's. */
 SW_API int  sw_batch_tune(sw_batch *b, void *stream, uint32_t max_trials, sw_batch_schedule *best,

@@ -74,6 +74,7 @@ struct sw_phantom {
     bool has_field = false;
     // bricked voxel records gathered by the walk (walk_args.hpp)
     void *rec[5] = {};               // by SW_FIELD_*: none (label only), f32, f16, bf16, q16
+    void *rec_soa = nullptr;         // float32 structure-of-arrays bricks (production walks)
 };
 
 struct sw_sequence {
@@ -377,7 +378,7 @@ static size_t phantom_bytes(uint64_t n, bool has_field, uint32_t formats) {
     if (!has_field) return need + n * 4;
     need += n * 4;
     for (int f = SW_FIELD_F32; f <= SW_FIELD_Q16; f++)
-        if (formats & (1u << f)) need += n * (f == SW_FIELD_F32 ? 8 : 4);
+        if (formats & (1u << f)) need += n * (f == SW_FIELD_F32 ? 8 + 6 : 4);   // f32: AoS and SoA bricks
     return need;
 }
 
@@ -418,6 +419,14 @@ static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
         const int kind = f == SW_FIELD_F32 ? 0 : f == SW_FIELD_F16 ? 1 : f == SW_FIELD_BF16 ? 2 : f == SW_FIELD_Q16 ? 4 : 3;
         SW_HIP(e, sw::launch_records(ph->mask, ph->field, nx, ny, nz, kind,
                                      1.0f / (f == SW_FIELD_Q16 ? ph->q_scale : ph->half_scale), ph->rec[f], s));
+        if (f == SW_FIELD_F32) {         // the production walk's 5-byte structure-of-arrays copy
+            const size_t bytes = (size_t)(sw::soa_brick_count(nx, sw::kSoaF32.bx) *
+                                          sw::soa_brick_count(ny, sw::kSoaF32.by) *
+                                          sw::soa_brick_count(nz, sw::kSoaF32.bz)) * 128;
+            SW_HIP(e, hipMalloc(&ph->rec_soa, bytes));
+            SW_HIP(e, hipMemsetAsync(ph->rec_soa, 0, bytes, s));
+            SW_HIP(e, sw::launch_soa_records(ph->mask, ph->field, nx, ny, nz, ph->rec_soa, s));
+        }
     }
     SW_HIP(e, hipStreamSynchronize(s));
     return SW_OK;
@@ -468,6 +477,7 @@ SW_API void sw_phantom_destroy(sw_phantom *ph) {
     dev_free(ph->mask);
     dev_free(ph->field);
     for (void *r : ph->rec) dev_free(r);
+    dev_free(ph->rec_soa);
     delete ph;
 }
 
@@ -715,9 +725,10 @@ namespace {
 // neighbouring voxels, and the XCD-aware block mapping of the walk kernel gives
 // each XCD (own L2) a compact region of the phantom.
 // Writes the visiting order of a's spins into out[0, n_local).
-int build_order_into(sw_engine *e, const sw::WalkArgs &a, uint32_t *out, hipStream_t s) {
+int build_order_into(sw_engine *e, const sw::WalkArgs &a, uint32_t *out, hipStream_t s,
+                     int lb = sw::kOrderBrickLog2) {
     const uint32_t n = a.n_local;
-    const sw::OrderPlan plan = sw::order_plan(n, a.nx, a.ny, a.nz);
+    const sw::OrderPlan plan = sw::order_plan(n, a.nx, a.ny, a.nz, lb);
     if (plan.total_bytes > e->sort_bytes) {
         SW_HIP(e, hipStreamSynchronize(s));
         dev_free(e->d_sort);
@@ -775,7 +786,13 @@ int fill_walk_args(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, con
 
     std::memset(&a, 0, sizeof a);
     a.rec = ph->rec[field];
-    {
+    if (field == SW_FIELD_F32 && o->rng == SW_RNG_PHILOX) {
+        // production walks gather the float32 field from the structure-of-arrays bricks
+        if (!ph->rec_soa) return fail(e, SW_E_INVALID, "the phantom has no float32 structure-of-arrays records");
+        a.rec = ph->rec_soa;
+        a.nb_y = (uint32_t)sw::soa_brick_count(ph->dims[1], sw::kSoaF32.by);
+        a.nb_z = (uint32_t)sw::soa_brick_count(ph->dims[2], sw::kSoaF32.bz);
+    } else {
         const sw::BrickShape sh = field == SW_FIELD_F32 ? sw::kBrick8 : sw::kBrick4;
         a.nb_y = (uint32_t)sw::brick_count(ph->dims[1], sh.lby);
         a.nb_z = (uint32_t)sw::brick_count(ph->dims[2], sh.lbz);
@@ -963,18 +980,40 @@ struct sw_batch {
     // schedule waits for the last run before it rewrites the chunk table.
     hipEvent_t run_ev = nullptr;
     hipStream_t run_stream = nullptr;
+    // Visiting orders: each ordered tile's arguments and its slice of d_order,
+    // rebuilt in place when a schedule changes the order brick (order_lb).
+    std::vector<std::pair<sw::WalkArgs, uint32_t *>> orders;
+    int order_lb = sw::kOrderBrickLog2;
+    size_t lds_cu = 0, lds_block = 0;            // LDS per CU and per workgroup, bytes
 };
 
 namespace {
 
 // Counter reset, the walk of every group and the echo sums, in stream order.
+// The LDS of a batch workgroup: the per-axis tables, raised so that at most
+// waves_per_simd waves fit each SIMD when the schedule asks for a cap.  A CU
+// has 4 SIMDs and a workgroup chunk_spins / 64 waves, so the cap is
+// k = 256 * waves_per_simd / chunk_spins workgroups per CU; k fit when
+// k * L <= LDS per CU < (k + 1) * L, so L = LDS per CU / (k + 1/2).
+size_t batch_lds(const sw_batch &b, const sw_batch_schedule &sc) {
+    size_t lds = b.idx32 ? b.lds : 0;
+    if (sc.waves_per_simd && b.lds_cu && sc.chunk_spins) {
+        const size_t k = (size_t)256 * sc.waves_per_simd / sc.chunk_spins;
+        const size_t l = (size_t)(2 * b.lds_cu / (2 * k + 1)) & ~(size_t)1023;
+        lds = std::max(lds, l);
+    }
+    return lds;
+}
+
 hipError_t batch_enqueue(const sw_batch *b, hipStream_t s) {
     hipError_t err = sw::launch_zero_u64(b->d_counters, (int)(4 * b->n_items), s);
     if (err != hipSuccess) return err;
+    const size_t lds = batch_lds(*b, b->sched);
     for (const auto &g : b->groups) {
         const void *items = g.views == 1 ? (const void *)b->d_items : (const void *)b->d_fused;
-        err = sw::launch_walk_batch(items, b->d_chunks + g.offset, g.n_slots, b->sched.chunk_spins, g.views, b->rng,
-                                    b->field, b->prec, g.cross, b->idx32, b->lds, s);
+        err = sw::launch_walk_batch(items, b->d_chunks + g.offset, g.n_slots, b->sched.chunk_spins,
+                                    b->sched.spins_per_lane, g.views, b->rng, b->field, b->prec, g.cross, b->idx32,
+                                    lds, s);
         if (err != hipSuccess) return err;
     }
     for (const auto &r : b->reduces) {
@@ -994,7 +1033,7 @@ hipError_t batch_enqueue(const sw_batch *b, hipStream_t s) {
 // 163.5 ms tile after tile, 151.2 ms with 16 bands, 155.2 ms with 64; with
 // fused walks the tuner picks 4 bands for FP32 (70.4 ms against 72.2 ms at 16)
 // and 1 for Kahan + int16 (66.5 against 68.3).  sw_batch_tune searches around it.
-constexpr sw_batch_schedule kDefaultSchedule = {256u, 1u << 20, 4u};
+constexpr sw_batch_schedule kDefaultSchedule = {256u, 1u << 20, 4u, 1u, 0u, 8u};
 
 // The chunk table of a schedule: per CROSS_FOV group and walk kind (single or
 // fused), each walk unit's chunks of sc.chunk_spins spins split into 8
@@ -1012,7 +1051,7 @@ void build_chunk_table(const sw_batch &b, const sw_batch_schedule &sc, std::vect
     table.clear();
     groups.clear();
     *kinds = 0;
-    const uint32_t cs = sc.chunk_spins;
+    const uint32_t cs = sc.chunk_spins * sc.spins_per_lane;
     for (int grp = 0; grp < 4; grp++) {
         const int cross = grp & 1, views = grp < 2 ? 1 : sw::kMaxViews;
         std::vector<uint32_t> idx;
@@ -1078,7 +1117,15 @@ void build_chunk_table(const sw_batch &b, const sw_batch_schedule &sc, std::vect
 
 bool schedule_ok(const sw_batch_schedule &sc) {
     return (sc.chunk_spins == 64 || sc.chunk_spins == 128 || sc.chunk_spins == 256) && sc.bands_shared >= 1 &&
-           sc.bands_mixed >= 1;
+           sc.bands_mixed >= 1 && (sc.spins_per_lane == 1 || sc.spins_per_lane == 2) &&
+           (sc.waves_per_simd == 0 || (sc.waves_per_simd >= 2 && sc.waves_per_simd <= 8)) &&
+           (sc.order_brick == 4 || sc.order_brick == 8 || sc.order_brick == 16 || sc.order_brick == 32);
+}
+
+int log2_of(uint32_t v) {
+    int l = 0;
+    while ((1u << l) < v) l++;
+    return l;
 }
 
 // Makes sc the batch's schedule: builds and uploads its chunk table and, when
@@ -1089,9 +1136,22 @@ int apply_schedule(sw_engine *e, sw_batch *b, const sw_batch_schedule &sc, hipSt
     std::vector<sw_batch::Group> groups;
     unsigned kinds = 0;
     build_chunk_table(*b, sc, table, groups, &kinds);
-    // the previous table may be in use by a run on any stream
+    if (batch_lds(*b, sc) > b->lds_block)
+        return fail(e, SW_E_INVALID, "waves_per_simd " + std::to_string(sc.waves_per_simd) +
+                                         " needs more LDS per workgroup than the device allows");
+    // the previous table and orders may be in use by a run on any stream
     if (b->run_stream) SW_HIP(e, hipEventSynchronize(b->run_ev));
     SW_HIP(e, hipStreamSynchronize(s));
+    const int lb = log2_of(sc.order_brick);
+    if (lb != b->order_lb && !b->orders.empty()) {
+        int rc = scratch_acquire(e, s);              // build_order_into uses the engine's sort scratch
+        if (rc) return rc;
+        for (const auto &o : b->orders)
+            if ((rc = build_order_into(e, o.first, o.second, s, lb))) return rc;
+        if ((rc = scratch_release(e, s))) return rc;
+        SW_HIP(e, hipStreamSynchronize(s));
+    }
+    b->order_lb = lb;
     if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
     if (table.size() > b->chunks_cap) {
@@ -1208,7 +1268,7 @@ SW_API int sw_batch_create(sw_engine *e, const sw_batch_item *items, uint32_t n_
         if (i == 0) b->field = field;
         else if (field != b->field)
             return fail(e, SW_E_INVALID, "tile " + std::to_string(i) + ": every tile of a batch gathers the same field format");
-        const sw::WalkMode m = sw::walk_mode(args[i], field);
+        const sw::WalkMode m = sw::walk_mode(args[i], field, b->rng);
         b->idx32 = b->idx32 && m.idx32;
         b->lds = std::max(b->lds, m.lds);
         exc_off[i] = exc.size();
@@ -1244,6 +1304,7 @@ SW_API int sw_batch_create(sw_engine *e, const sw_batch_item *items, uint32_t n_
             uint32_t *dst = b->d_order + order_off[i];
             int rc = build_order_into(e, a, dst, s);
             if (rc) return rc;
+            b->orders.push_back({a, dst});
             a.order = dst;
         }
     }
@@ -1295,6 +1356,13 @@ SW_API int sw_batch_create(sw_engine *e, const sw_batch_item *items, uint32_t n_
     SW_HIP(e, hipEventCreate(&b->ev0));
     SW_HIP(e, hipEventCreate(&b->ev1));
     SW_HIP(e, hipEventCreateWithFlags(&b->run_ev, hipEventDisableTiming));
+    {
+        int v = 0;
+        SW_HIP(e, hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, e->device));
+        b->lds_cu = (size_t)v;
+        SW_HIP(e, hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device));
+        b->lds_block = (size_t)v;
+    }
     int rc = apply_schedule(e, b.get(), kDefaultSchedule, s, true);
     if (rc) return rc;
     *out = b.release();
@@ -1371,17 +1439,19 @@ SW_API int sw_batch_set_schedule(sw_batch *b, const sw_batch_schedule *sc) {
     if (!b) return SW_E_INVALID;
     sw_engine *e = b->eng;
     if (!sc || !schedule_ok(*sc))
-        return fail(e, SW_E_INVALID, "schedule needs chunk_spins 64, 128 or 256 and at least one band");
+        return fail(e, SW_E_INVALID, "schedule needs chunk_spins 64, 128 or 256, at least one band, spins_per_lane 1 "
+                                     "or 2, waves_per_simd 0 or 2-8 and order_brick 4, 8, 16 or 32");
     SW_HIP(e, hipSetDevice(b->device));
     return apply_schedule(e, b, *sc, e->stream, true);
 }
 
 // Coordinate search over the schedule: starting from the current one, each
-// parameter in turn (chunk size, then the band counts of the class kinds the
-// batch has) takes the candidate value that times fastest, a candidate having to
-// beat the best by 1 %.  Each trial is one full run of the batch, timed with HIP
-// events; schedules change only the order in which chunks are walked, so every
-// trial leaves the batch's outputs as any run does.
+// parameter in turn (chunk size, the band counts of the class kinds the batch
+// has, spins per lane, resident waves per SIMD, visiting-order brick) takes
+// the candidate value that times fastest, a candidate having to beat the best
+// by 1 %.  Each trial is one full run of the batch, timed with HIP events;
+// schedules change only which spins run where and when, so every trial leaves
+// the batch's outputs as any run does.
 SW_API int sw_batch_tune(sw_batch *b, void *stream, uint32_t max_trials, sw_batch_schedule *best_out,
                          float *best_ms, float *first_ms) {
     if (!b) return SW_E_INVALID;
@@ -1413,9 +1483,8 @@ SW_API int sw_batch_tune(sw_batch *b, void *stream, uint32_t max_trials, sw_batc
     const unsigned kinds = schedule_kinds(*b);
     auto consider = [&](sw_batch_schedule cand) -> int {
         if (max_trials && trials >= max_trials) return SW_OK;
-        if (cand.chunk_spins == best.chunk_spins && cand.bands_shared == best.bands_shared &&
-            cand.bands_mixed == best.bands_mixed)
-            return SW_OK;
+        if (std::memcmp(&cand, &best, sizeof cand) == 0) return SW_OK;
+        if (batch_lds(*b, cand) > b->lds_block) return SW_OK;     // a cap this device cannot express
         int r = timed(cand, &t);
         if (r) return r;
         if (t < 0.99f * t_best) { best = cand; t_best = t; }
@@ -1435,6 +1504,18 @@ SW_API int sw_batch_tune(sw_batch *b, void *stream, uint32_t max_trials, sw_batc
             sw_batch_schedule x = best; x.bands_mixed = v;
             if ((rc = consider(x))) return rc;
         }
+    for (uint32_t v : {2u, 1u}) {
+        sw_batch_schedule x = best; x.spins_per_lane = v;
+        if ((rc = consider(x))) return rc;
+    }
+    for (uint32_t v : {5u, 4u, 3u, 0u}) {
+        sw_batch_schedule x = best; x.waves_per_simd = v;
+        if ((rc = consider(x))) return rc;
+    }
+    for (uint32_t v : {16u, 4u, 32u, 8u}) {
+        sw_batch_schedule x = best; x.order_brick = v;
+        if ((rc = consider(x))) return rc;
+    }
     rc = apply_schedule(e, b, best, s, true);
     if (rc) return rc;
     if (best_out) *best_out = best;

@@ -11,10 +11,10 @@ hipError_t launch_walk(const WalkArgs &a, int rng, int field, int prec, hipStrea
     return hipErrorInvalidValue;
 }
 
-WalkMode walk_mode(const WalkArgs &a, int field) {
+WalkMode walk_mode(const WalkArgs &a, int field, int rng) {
     switch (field) {
     case 0: return mode_of<RecNone>(a);
-    case 1: return mode_of<RecF32>(a);
+    case 1: return rng == 1 ? mode_of<RecF32>(a) : mode_of<RecF32S>(a);
     case 2: return mode_of<RecF16>(a);
     case 4: return mode_of<RecQ16>(a);
     default: return mode_of<RecBF16>(a);
@@ -22,12 +22,14 @@ WalkMode walk_mode(const WalkArgs &a, int field) {
 }
 
 hipError_t launch_walk_batch(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                             int views, int rng, int field, int prec, bool cross, bool idx32, size_t lds,
-                             hipStream_t s) {
+                             uint32_t spl, int views, int rng, int field, int prec, bool cross, bool idx32,
+                             size_t lds, hipStream_t s) {
     if (n_slots == 0) return hipSuccess;
-    if (n_slots % 8 || threads == 0 || threads > 256 || threads % 64) return hipErrorInvalidValue;
-    if (rng == 1) return launch_walk_batch_replay(items, chunks, n_slots, threads, views, field, prec, cross, idx32, lds, s);
-    if (rng == 0) return launch_walk_batch_philox(items, chunks, n_slots, threads, views, field, prec, cross, idx32, lds, s);
+    if (n_slots % 8 || threads == 0 || threads > 256 || threads % 64 || spl == 0) return hipErrorInvalidValue;
+    if (rng == 1)
+        return launch_walk_batch_replay(items, chunks, n_slots, threads, spl, views, field, prec, cross, idx32, lds, s);
+    if (rng == 0)
+        return launch_walk_batch_philox(items, chunks, n_slots, threads, spl, views, field, prec, cross, idx32, lds, s);
     return hipErrorInvalidValue;
 }
 

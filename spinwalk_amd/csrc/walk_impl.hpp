@@ -145,6 +145,7 @@ __device__ __forceinline__ T load_rec(const void *base, uint64_t i) {
 
 struct RecF32 {                          // {float field_T, uint32 label}
     static constexpr BrickShape shape = kBrick8;
+    static constexpr bool kSoA = false;
     using T = uint2;
     __device__ static uint8_t label(T r) { return (uint8_t)r.y; }
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
@@ -154,12 +155,14 @@ struct RecF32 {                          // {float field_T, uint32 label}
 };
 struct RecNone {                         // label only: {0, uint8 label, pad}
     static constexpr BrickShape shape = kBrick4;
+    static constexpr bool kSoA = false;
     using T = uint32_t;
     __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
     template <class R> __device__ static R field(const WalkArgs &, T) { return (R)0; }
 };
 struct RecF16 {                          // {fp16 field_T/half_scale, uint8 label, pad}
     static constexpr BrickShape shape = kBrick4;
+    static constexpr bool kSoA = false;
     using T = uint32_t;
     __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
@@ -168,6 +171,7 @@ struct RecF16 {                          // {fp16 field_T/half_scale, uint8 labe
 };
 struct RecQ16 {                          // {int16 rint(field_T/q_scale), uint8 label, pad}
     static constexpr BrickShape shape = kBrick4;
+    static constexpr bool kSoA = false;
     using T = uint32_t;
     __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
@@ -176,12 +180,38 @@ struct RecQ16 {                          // {int16 rint(field_T/q_scale), uint8 
 };
 struct RecBF16 {                         // {bf16 field_T/half_scale, uint8 label, pad}
     static constexpr BrickShape shape = kBrick4;
+    static constexpr bool kSoA = false;
     using T = uint32_t;
     __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
         return (R)(__uint_as_float(r << 16) * a.h2d);
     }
 };
+
+// The production walk's float32 records: structure-of-arrays bricks of 2x3x4
+// voxels (kSoaF32).  A record index is the byte offset of the voxel's label;
+// its field sits 24 + 3 j bytes further, j = the slot in the brick = offset
+// mod 128.  Two loads from one line.
+struct RecF32S {
+    static constexpr bool kSoA = true;
+    static constexpr SoaShape soa = kSoaF32;
+    using T = uint2;                     // {field bits, label}, as RecF32
+    __device__ static uint8_t label(T r) { return (uint8_t)r.y; }
+    template <class R> __device__ static R field(const WalkArgs &a, T r) { return RecF32::field<R>(a, r); }
+    template <bool IDX32, class I> __device__ static T load(const void *base, I off) {
+        const char *b = (const char *)base;
+        T r;
+        r.y = *(const uint8_t *)(b + off);
+        r.x = *(const uint32_t *)(b + off + (I)soa.nvox + (I)(soa.field_bytes - 1) * (off & (I)127));
+        return r;
+    }
+};
+
+template <class Rec, bool IDX32, class I>
+__device__ __forceinline__ typename Rec::T rec_load(const void *base, I off) {
+    if constexpr (Rec::kSoA) return Rec::template load<IDX32>(base, off);
+    else return load_rec<typename Rec::T, IDX32>(base, off);
+}
 
 // ---------------------------------------------------------------------------
 // Draw sources.
@@ -263,31 +293,52 @@ template <bool IDX32> struct Vox {
 // axis_tab[0, nx) = X, [nx, nx+ny) = Y, [nx+ny, nx+ny+nz) = Z.
 extern __shared__ uint32_t axis_tab[];
 
+// For the structure-of-arrays bricks the index is the label's byte offset,
+// separable in the same way: X = (ix / bx) * nb_y * nb_z * 128 + (ix mod bx) * by * bz,
+// Y = (iy / by) * nb_z * 128 + (iy mod by) * bz, Z = (iz / bz) * 128 + iz mod bz.
+template <class Rec, class I>
+__device__ __forceinline__ I soa_part(const WalkArgs &a, int axis, I i) {
+    constexpr SoaShape S = Rec::soa;
+    if (axis == 0) return (i / S.bx) * (I)a.nb_y * (I)a.nb_z * 128 + (i % S.bx) * (S.by * S.bz);
+    if (axis == 1) return (i / S.by) * (I)a.nb_z * 128 + (i % S.by) * S.bz;
+    return (i / S.bz) * 128 + i % S.bz;
+}
+
 template <class Rec>
 __device__ __forceinline__ uint32_t axis_entry(const WalkArgs &a, uint32_t i) {
-    constexpr int LX = Rec::shape.lbx, LY = Rec::shape.lby, LZ = Rec::shape.lbz, L = LX + LY + LZ;
     const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny;
-    if (i < nx)
-        return (((i >> LX) * a.nb_y * a.nb_z) << L) | ((i & ((1u << LX) - 1)) << (LY + LZ));
-    if (i < nx + ny) {
-        i -= nx;
-        return (((i >> LY) * a.nb_z) << L) | ((i & ((1u << LY) - 1)) << LZ);
+    if constexpr (Rec::kSoA) {
+        if (i < nx) return soa_part<Rec, uint32_t>(a, 0, i);
+        if (i < nx + ny) return soa_part<Rec, uint32_t>(a, 1, i - nx);
+        return soa_part<Rec, uint32_t>(a, 2, i - nx - ny);
+    } else {
+        constexpr int LX = Rec::shape.lbx, LY = Rec::shape.lby, LZ = Rec::shape.lbz, L = LX + LY + LZ;
+        if (i < nx)
+            return (((i >> LX) * a.nb_y * a.nb_z) << L) | ((i & ((1u << LX) - 1)) << (LY + LZ));
+        if (i < nx + ny) {
+            i -= nx;
+            return (((i >> LY) * a.nb_z) << L) | ((i & ((1u << LY) - 1)) << LZ);
+        }
+        i -= nx + ny;
+        return ((i >> LZ) << L) | (i & ((1u << LZ) - 1));
     }
-    i -= nx + ny;
-    return ((i >> LZ) << L) | (i & ((1u << LZ) - 1));
 }
 
 template <class Rec, bool IDX32>
 __device__ __forceinline__ uint64_t record_of(const WalkArgs &a, uint64_t ix, uint64_t iy, uint64_t iz) {
-    constexpr int LX = Rec::shape.lbx, LY = Rec::shape.lby, LZ = Rec::shape.lbz;
     if (IDX32) {
         const uint32_t nx = (uint32_t)a.nx, nxy = nx + (uint32_t)a.ny;
         return axis_tab[(uint32_t)ix] + axis_tab[nx + (uint32_t)iy] + axis_tab[nxy + (uint32_t)iz];
     }
-    const uint64_t inner = ((ix & ((1u << LX) - 1)) << (LY + LZ)) | ((iy & ((1u << LY) - 1)) << LZ) |
-                           (iz & ((1u << LZ) - 1));
-    const uint64_t b = ((ix >> LX) * a.nb_y + (iy >> LY)) * a.nb_z + (iz >> LZ);
-    return (b << (LX + LY + LZ)) | inner;
+    if constexpr (Rec::kSoA) {
+        return soa_part<Rec, uint64_t>(a, 0, ix) + soa_part<Rec, uint64_t>(a, 1, iy) + soa_part<Rec, uint64_t>(a, 2, iz);
+    } else {
+        constexpr int LX = Rec::shape.lbx, LY = Rec::shape.lby, LZ = Rec::shape.lbz;
+        const uint64_t inner = ((ix & ((1u << LX) - 1)) << (LY + LZ)) | ((iy & ((1u << LY) - 1)) << LZ) |
+                               (iz & ((1u << LZ) - 1));
+        const uint64_t b = ((ix >> LX) * a.nb_y + (iy >> LY)) * a.nb_z + (iz >> LZ);
+        return (b << (LX + LY + LZ)) | inner;
+    }
 }
 
 // Truncating double -> int32 conversion with the instruction's defined
@@ -523,7 +574,7 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
         if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = P::metres(a, xo[i], i);
         return ST_LOST;
     }
-    uint8_t ts_old = Rec::label(load_rec<typename Rec::T, IDX32>(a.rec, vx.rec));   // :103
+    uint8_t ts_old = Rec::label(rec_load<Rec, IDX32>(a.rec, vx.rec));   // :103
     typename P::Step stp;
     stp.set(a, ts_old);                                            // :104
     // The reference re-reads T1/T2/diffusivity of ts_old at every voxel change
@@ -576,7 +627,7 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
             const bool newvox = vx.rec != rec_old;                 // :150
             typename Rec::T r{};
             if (newvox)                                            // one gather: label and field
-                r = load_rec<typename Rec::T, IDX32>(a.rec, vx.rec);
+                r = rec_load<Rec, IDX32>(a.rec, vx.rec);
             if (moved) {                                           // next attempt's draws while the load flies
                 rng.commit();
                 rng.peek();
@@ -667,11 +718,11 @@ done:
     return st;
 }
 
-// One workgroup walks the spins at positions [t0, t0 + blockDim.x) of the
-// launch's visiting order (fewer at the end).
+// One workgroup walks the spins at positions [t0, t0 + spl * blockDim.x) of the
+// launch's visiting order (fewer at the end): lane l takes positions
+// t0 + k * blockDim.x + l, k = 0 .. spl-1, one after the other.
 template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
-__device__ __forceinline__ void walk_block(const WalkArgs &a, uint32_t t0) {
-    const uint32_t t = t0 + threadIdx.x;
+__device__ __forceinline__ void walk_block(const WalkArgs &a, uint32_t t0, uint32_t spl = 1) {
     if (IDX32) {
         const uint32_t n_tab = (uint32_t)(a.nx + a.ny + a.nz);
         for (uint32_t i = threadIdx.x; i < n_tab; i += blockDim.x) axis_tab[i] = axis_entry<Rec>(a, i);
@@ -679,11 +730,14 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, uint32_t t0) {
     }
     uint64_t n_acc = 0;
     uint32_t n_rej = 0, lost = 0, exh = 0;
-    if (t < a.n_local) {
-        const uint32_t li = a.order ? a.order[t] : t;
-        const int st = walk_spin<Rng, Rec, Phase, CROSS, IDX32>(a, li, n_acc, n_rej);
-        lost = (st == ST_LOST);
-        exh = (st == ST_EXHAUSTED);
+    for (uint32_t k = 0; k < spl; k++) {
+        const uint32_t t = t0 + k * blockDim.x + threadIdx.x;
+        if (t < a.n_local) {
+            const uint32_t li = a.order ? a.order[t] : t;
+            const int st = walk_spin<Rng, Rec, Phase, CROSS, IDX32>(a, li, n_acc, n_rej);
+            lost += (st == ST_LOST);
+            exh += (st == ST_EXHAUSTED);
+        }
     }
     const uint64_t s_acc = wave_sum_u64(n_acc), s_rej = wave_sum_u64(n_rej);
     const uint64_t s_lost = wave_sum_u64(lost), s_exh = wave_sum_u64(exh);
@@ -752,7 +806,7 @@ __device__ int walk_spin_views(const WalkArgs &a, const WalkView *xv, uint32_t l
                 for (int i = 0; i < 3; i++) xyz1_of(v)[i] = P::metres(a, xo[i], i);
         return ST_LOST;
     }
-    uint8_t ts_old = Rec::label(load_rec<typename Rec::T, IDX32>(a.rec, vx.rec));   // :103
+    uint8_t ts_old = Rec::label(rec_load<Rec, IDX32>(a.rec, vx.rec));   // :103
     typename P::Step stp;
     stp.set(a, ts_old);                                            // :104
     // The reference re-reads T1/T2/diffusivity of ts_old at every voxel change
@@ -805,7 +859,7 @@ __device__ int walk_spin_views(const WalkArgs &a, const WalkView *xv, uint32_t l
             const bool newvox = vx.rec != rec_old;                 // :150
             typename Rec::T r{};
             if (newvox)                                            // one gather: label and field
-                r = load_rec<typename Rec::T, IDX32>(a.rec, vx.rec);
+                r = rec_load<Rec, IDX32>(a.rec, vx.rec);
             if (moved) {                                           // next attempt's draws while the load flies
                 rng.commit();
                 rng.peek();
@@ -916,8 +970,7 @@ done:
 // launch's visiting order (fewer at the end).
 template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32, int NV>
 __device__ __forceinline__ void walk_block_views(const WalkArgs &a, const WalkView *xv, uint32_t n_views,
-                                                 uint32_t t0) {
-    const uint32_t t = t0 + threadIdx.x;
+                                                 uint32_t t0, uint32_t spl) {
     if (IDX32) {
         const uint32_t n_tab = (uint32_t)(a.nx + a.ny + a.nz);
         for (uint32_t i = threadIdx.x; i < n_tab; i += blockDim.x) axis_tab[i] = axis_entry<Rec>(a, i);
@@ -925,11 +978,14 @@ __device__ __forceinline__ void walk_block_views(const WalkArgs &a, const WalkVi
     }
     uint64_t n_acc = 0;
     uint32_t n_rej = 0, lost = 0, exh = 0;
-    if (t < a.n_local) {
-        const uint32_t li = a.order ? a.order[t] : t;
-        const int st = walk_spin_views<Rng, Rec, Phase, CROSS, IDX32, NV>(a, xv, li, n_acc, n_rej);
-        lost = (st == ST_LOST);
-        exh = (st == ST_EXHAUSTED);
+    for (uint32_t k = 0; k < spl; k++) {
+        const uint32_t t = t0 + k * blockDim.x + threadIdx.x;
+        if (t < a.n_local) {
+            const uint32_t li = a.order ? a.order[t] : t;
+            const int st = walk_spin_views<Rng, Rec, Phase, CROSS, IDX32, NV>(a, xv, li, n_acc, n_rej);
+            lost += (st == ST_LOST);
+            exh += (st == ST_EXHAUSTED);
+        }
     }
     const uint64_t s_acc = wave_sum_u64(n_acc), s_rej = wave_sum_u64(n_rej);
     const uint64_t s_lost = wave_sum_u64(lost), s_exh = wave_sum_u64(exh);
@@ -983,15 +1039,15 @@ constexpr int walk_waves_per_eu() {
 
 template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32, int NV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(walk_waves_per_eu<Phase, NV>())))
-void walk_batch_kernel(const void *__restrict__ items, const BatchChunk *__restrict__ chunks) {
+void walk_batch_kernel(const void *__restrict__ items, const BatchChunk *__restrict__ chunks, uint32_t spl) {
     const uint32_t per = gridDim.x / 8, b = blockIdx.x;
     const BatchChunk c = chunks[(b % 8) * per + b / 8];
     if (c.item == kNoItem) return;                    // padding slot (uniform across the block)
     if constexpr (NV == 1) {
-        walk_block<Rng, Rec, Phase, CROSS, IDX32>(((const WalkArgs *)items)[c.item], c.first);
+        walk_block<Rng, Rec, Phase, CROSS, IDX32>(((const WalkArgs *)items)[c.item], c.first, spl);
     } else {
         const FusedArgs &f = ((const FusedArgs *)items)[c.item];
-        walk_block_views<Rng, Rec, Phase, CROSS, IDX32, NV>(f.a, f.xv, f.n_views, c.first);
+        walk_block_views<Rng, Rec, Phase, CROSS, IDX32, NV>(f.a, f.xv, f.n_views, c.first, spl);
     }
 }
 
@@ -999,11 +1055,16 @@ void walk_batch_kernel(const void *__restrict__ items, const BatchChunk *__restr
 // 64 KiB of LDS.
 template <class F>
 WalkMode mode_of(const WalkArgs &a) {
-    constexpr int lb = F::shape.lbx + F::shape.lby + F::shape.lbz;
-    const uint64_t records = brick_count(a.nx, F::shape.lbx) * a.nb_y * a.nb_z << lb;
+    uint64_t bytes;
+    if constexpr (F::kSoA) {
+        bytes = soa_brick_count(a.nx, F::soa.bx) * a.nb_y * a.nb_z * 128;
+    } else {
+        constexpr int lb = F::shape.lbx + F::shape.lby + F::shape.lbz;
+        bytes = (brick_count(a.nx, F::shape.lbx) * a.nb_y * a.nb_z << lb) * sizeof(typename F::T);
+    }
     const uint64_t n_tab = (uint64_t)a.nx + (uint64_t)a.ny + (uint64_t)a.nz;
     WalkMode m;
-    m.idx32 = records * sizeof(typename F::T) < (1ull << 32) && n_tab * 4 <= 65536;
+    m.idx32 = bytes < (1ull << 32) && n_tab * 4 <= 65536;
     m.lds = m.idx32 ? (size_t)n_tab * 4 : 0;
     return m;
 }
@@ -1025,14 +1086,14 @@ hipError_t launch_fp(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
 
 template <class Rng, class F, class P, int NV>
 hipError_t launch_batch_nv(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                           bool cross, bool idx32, size_t lds, hipStream_t s) {
+                           uint32_t spl, bool cross, bool idx32, size_t lds, hipStream_t s) {
     const dim3 grid(n_slots), block(threads);
     if (cross) {
-        if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, true, NV>), grid, block, lds, s, items, chunks);
-        else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, false, NV>), grid, block, 0, s, items, chunks);
+        if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, true, NV>), grid, block, lds, s, items, chunks, spl);
+        else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, false, NV>), grid, block, lds, s, items, chunks, spl);
     } else {
-        if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, false, true, NV>), grid, block, lds, s, items, chunks);
-        else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, false, false, NV>), grid, block, 0, s, items, chunks);
+        if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, false, true, NV>), grid, block, lds, s, items, chunks, spl);
+        else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, false, false, NV>), grid, block, lds, s, items, chunks, spl);
     }
     return hipGetLastError();
 }
@@ -1040,33 +1101,38 @@ hipError_t launch_batch_nv(const void *items, const BatchChunk *chunks, uint32_t
 // views: 1, or kMaxViews for fused items (FP32 and Kahan phases only).
 template <class Rng, class F, class P>
 hipError_t launch_batch_fp(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                           int views, bool cross, bool idx32, size_t lds, hipStream_t s) {
-    if (views == 1) return launch_batch_nv<Rng, F, P, 1>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+                           uint32_t spl, int views, bool cross, bool idx32, size_t lds, hipStream_t s) {
+    if (views == 1) return launch_batch_nv<Rng, F, P, 1>(items, chunks, n_slots, threads, spl, cross, idx32, lds, s);
     if constexpr (sizeof(typename P::R) == 4) {
         if (views == kMaxViews)
-            return launch_batch_nv<Rng, F, P, kMaxViews>(items, chunks, n_slots, threads, cross, idx32, lds, s);
+            return launch_batch_nv<Rng, F, P, kMaxViews>(items, chunks, n_slots, threads, spl, cross, idx32, lds, s);
     }
     return hipErrorInvalidValue;
 }
 
+// The float32 records a walk gathers: the reference layout for replayed draws
+// (held bit for bit to the oracle), the structure-of-arrays bricks in production.
+template <class Rng> using RecF32Of = typename std::conditional<Rng::kReplay, RecF32, RecF32S>::type;
+
 template <class Rng>
 hipError_t launch_batch_rng(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                            int views, int field, int prec, bool cross, bool idx32, size_t lds, hipStream_t s) {
+                            uint32_t spl, int views, int field, int prec, bool cross, bool idx32, size_t lds,
+                            hipStream_t s) {
     if (prec == 2) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF64>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF64>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF64>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32Of<Rng>, PhaseF64>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
     } else if (prec == 1) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseKahan>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32Of<Rng>, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
     } else if (prec == 0) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
-        if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseF32>(items, chunks, n_slots, threads, views, cross, idx32, lds, s);
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32Of<Rng>, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1078,16 +1144,16 @@ hipError_t launch_rng(const WalkArgs &a, int field, int prec, hipStream_t s) {
     const dim3 block(256), grid((nb + 7) / 8 * 8);
     if (prec == 2) {
         if (field == 0) return launch_fp<Rng, RecNone, PhaseF64>(a, s, grid, block);
-        if (field == 1) return launch_fp<Rng, RecF32, PhaseF64>(a, s, grid, block);
+        if (field == 1) return launch_fp<Rng, RecF32Of<Rng>, PhaseF64>(a, s, grid, block);
     } else if (prec == 1) {
         if (field == 0) return launch_fp<Rng, RecNone, PhaseKahan>(a, s, grid, block);
-        if (field == 1) return launch_fp<Rng, RecF32, PhaseKahan>(a, s, grid, block);
+        if (field == 1) return launch_fp<Rng, RecF32Of<Rng>, PhaseKahan>(a, s, grid, block);
         if (field == 2) return launch_fp<Rng, RecF16, PhaseKahan>(a, s, grid, block);
         if (field == 3) return launch_fp<Rng, RecBF16, PhaseKahan>(a, s, grid, block);
         if (field == 4) return launch_fp<Rng, RecQ16, PhaseKahan>(a, s, grid, block);
     } else if (prec == 0) {
         if (field == 0) return launch_fp<Rng, RecNone, PhaseF32>(a, s, grid, block);
-        if (field == 1) return launch_fp<Rng, RecF32, PhaseF32>(a, s, grid, block);
+        if (field == 1) return launch_fp<Rng, RecF32Of<Rng>, PhaseF32>(a, s, grid, block);
         if (field == 2) return launch_fp<Rng, RecF16, PhaseF32>(a, s, grid, block);
         if (field == 3) return launch_fp<Rng, RecBF16, PhaseF32>(a, s, grid, block);
         if (field == 4) return launch_fp<Rng, RecQ16, PhaseF32>(a, s, grid, block);
@@ -1101,10 +1167,10 @@ hipError_t launch_rng(const WalkArgs &a, int field, int prec, hipStream_t s) {
 hipError_t launch_walk_replay(const WalkArgs &a, int field, int prec, hipStream_t s);
 hipError_t launch_walk_philox(const WalkArgs &a, int field, int prec, hipStream_t s);
 hipError_t launch_walk_batch_replay(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                                    int views, int field, int prec, bool cross, bool idx32, size_t lds,
+                                    uint32_t spl, int views, int field, int prec, bool cross, bool idx32, size_t lds,
                                     hipStream_t s);
 hipError_t launch_walk_batch_philox(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                                    int views, int field, int prec, bool cross, bool idx32, size_t lds,
+                                    uint32_t spl, int views, int field, int prec, bool cross, bool idx32, size_t lds,
                                     hipStream_t s);
 
 }  // namespace sw

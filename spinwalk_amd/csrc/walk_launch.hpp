@@ -15,16 +15,17 @@ hipError_t launch_walk(const WalkArgs &a, int rng, int field, int prec, hipStrea
 // The index mode the walk takes for a phantom and field format: 32-bit record
 // offsets with per-axis LDS tables of `lds` bytes, or 64-bit arithmetic.
 struct WalkMode { bool idx32; size_t lds; };
-WalkMode walk_mode(const WalkArgs &a, int field);
+WalkMode walk_mode(const WalkArgs &a, int field, int rng);
 
 // The batched walk over a chunk table of n_slots entries (a multiple of 8),
-// one workgroup of `threads` lanes (64, 128 or 256) per chunk of as many spins:
+// one workgroup of `threads` lanes (64, 128 or 256) per chunk of threads * spl
+// spins (each lane walks spl spins, one after the other):
 // items[] (WalkArgs, or FusedArgs when views == kMaxViews; prec 0 or 1 only)
 // and chunks[] in device memory; every item shares rng, field, prec, cross and
 // the index mode (idx32 with `lds` bytes covering every item's tables).
 hipError_t launch_walk_batch(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                             int views, int rng, int field, int prec, bool cross, bool idx32, size_t lds,
-                             hipStream_t s);
+                             uint32_t spl, int views, int rng, int field, int prec, bool cross, bool idx32,
+                             size_t lds, hipStream_t s);
 
 // Production draws of spins [spin0, spin0+n_spins), attempts [0, n_att).
 hipError_t launch_philox_draws(uint64_t seed, uint32_t spin0, uint32_t n_spins, uint32_t n_att,
@@ -46,6 +47,10 @@ hipError_t launch_phantom_stats(const uint8_t *mask, const float *field, uint64_
 // padding.
 hipError_t launch_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz, int kind,
                           float inv_scale, void *out, hipStream_t s);
+// Row-major mask/field -> the structure-of-arrays float32 bricks (kSoaF32) of
+// the production walk; `out` must be zero-filled for padding.
+hipError_t launch_soa_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz,
+                              void *out, hipStream_t s);
 // FP64 echo reduction into out[(te*n_sub+label)*3 + {0,1,2}]; partial is scratch
 // of reduce_partial_len(n_spins, n_te, n_sub) doubles.
 uint64_t reduce_partial_len(uint32_t n_spins, uint32_t n_te, uint32_t n_sub);
@@ -63,16 +68,19 @@ hipError_t noalloc_scratch_init(int device, void **addr);
 hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
 
 // The spatial visiting order (sort.hip): order[k] = the spin walked k-th, sorted
-// by the Morton index of its start voxel's 8^3 brick.  order_plan sizes the
-// scratch (total_bytes) and picks a counting sort or a radix sort.
+// by the Morton index of the (2^lb)^3 voxel brick of its start position (lb 3
+// by default).  order_plan sizes the scratch (total_bytes) and picks a counting
+// sort or a radix sort.
+constexpr int kOrderBrickLog2 = 3;
 struct OrderPlan {
+    uint32_t lb;            // brick edge 2^lb voxels
     bool counting;
     uint32_t bits;          // Morton bits per axis
     uint32_t bins;          // counting sort: 2^(3 bits)
     size_t temp_bytes;      // hipCUB scan or sort temp
     size_t total_bytes;     // scratch needed
 };
-OrderPlan order_plan(uint32_t n, int64_t nx, int64_t ny, int64_t nz);
+OrderPlan order_plan(uint32_t n, int64_t nx, int64_t ny, int64_t nz, int lb = kOrderBrickLog2);
 hipError_t launch_spatial_order(const OrderPlan &p, void *scratch, const float *XYZ0, float xyz_scale, uint32_t n,
                                 const double *s2g, int64_t nx, int64_t ny, int64_t nz, uint32_t *order,
                                 hipStream_t s);

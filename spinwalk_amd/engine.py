@@ -72,10 +72,11 @@ class SwWalkOpts(C.Structure):
 
 
 class SwBatchSchedule(C.Structure):
-    _fields_ = [("chunk_spins", C.c_uint32), ("bands_shared", C.c_uint32), ("bands_mixed", C.c_uint32)]
+    _fields_ = [("chunk_spins", C.c_uint32), ("bands_shared", C.c_uint32), ("bands_mixed", C.c_uint32),
+                ("spins_per_lane", C.c_uint32), ("waves_per_simd", C.c_uint32), ("order_brick", C.c_uint32)]
 
     def as_dict(self):
-        return dict(chunk_spins=self.chunk_spins, bands_shared=self.bands_shared, bands_mixed=self.bands_mixed)
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class SwBatchItem(C.Structure):
@@ -672,8 +673,9 @@ class Batch:
         self.eng._check(self.eng.L.sw_batch_get_schedule(self.h, C.byref(sc)), "sw_batch_get_schedule")
         return sc.as_dict()
 
-    def set_schedule(self, chunk_spins, bands_shared, bands_mixed):
-        sc = SwBatchSchedule(int(chunk_spins), int(bands_shared), int(bands_mixed))
+    def set_schedule(self, chunk_spins, bands_shared, bands_mixed, spins_per_lane=1, waves_per_simd=0, order_brick=8):
+        sc = SwBatchSchedule(int(chunk_spins), int(bands_shared), int(bands_mixed), int(spins_per_lane),
+                             int(waves_per_simd), int(order_brick))
         self.eng._check(self.eng.L.sw_batch_set_schedule(self.h, C.byref(sc)), "sw_batch_set_schedule")
 
     def tune(self, max_trials=0, stream=None):

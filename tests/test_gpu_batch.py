@@ -167,7 +167,8 @@ def test_batch_schedules_and_autotuner_keep_results(engine):
     tiles += [_tile(engine, torch, c, si, **kw) for c, si, kw in (cs[2], cs[3])]
     refs = [_separate(engine, torch, t) for t in tiles]
     b = engine.batch(tiles)
-    assert b.schedule == dict(chunk_spins=256, bands_shared=1 << 20, bands_mixed=4)
+    keys = ("chunk_spins", "bands_shared", "bands_mixed", "spins_per_lane", "waves_per_simd", "order_brick")
+    assert b.schedule == dict(zip(keys, (256, 1 << 20, 4, 1, 0, 8)))
 
     def check_run():
         _clear(tiles)
@@ -176,18 +177,25 @@ def test_batch_schedules_and_autotuner_keep_results(engine):
         for t, r, x in zip(tiles, refs, st["items"]):
             _check(t, r, x)
 
-    for sc in ((128, 64, 4), (64, 1, 1), (256, 16, 64), (128, 1 << 20, 16)):
+    # chunk sizes and bands; two spins per lane; caps of 2 to 4 waves per SIMD;
+    # visiting orders of 4^3, 16^3 and 32^3 voxel bricks (rebuilt in place)
+    for sc in ((128, 64, 4), (64, 1, 1), (256, 16, 64), (128, 1 << 20, 16), (256, 1 << 20, 4, 2),
+               (64, 4, 4, 2, 3), (256, 1 << 20, 4, 1, 2, 4), (128, 16, 16, 2, 0, 16), (256, 1 << 20, 4, 1, 4, 32),
+               (256, 1 << 20, 4, 1, 0, 8)):
         b.set_schedule(*sc)
-        assert b.schedule == dict(chunk_spins=sc[0], bands_shared=sc[1], bands_mixed=sc[2])
+        want = dict(zip(keys, (256, 1 << 20, 4, 1, 0, 8)))
+        want.update(zip(keys, sc))
+        assert b.schedule == want
+        assert b.graph_info()["uses_graph"]
         check_run()
-    res = b.tune(max_trials=8)
+    res = b.tune(max_trials=16)
     assert res["schedule"] == b.schedule and res["best_ms"] > 0 and res["first_ms"] > 0
     assert res["schedule"]["chunk_spins"] in (64, 128, 256)
     check_run()
-    with pytest.raises(E.EngineError, match="chunk_spins"):
-        b.set_schedule(100, 1, 1)
-    with pytest.raises(E.EngineError, match="chunk_spins"):
-        b.set_schedule(256, 0, 1)
+    for bad in ((100, 1, 1), (256, 0, 1), (256, 1, 1, 3), (256, 1, 1, 1, 1), (256, 1, 1, 1, 9),
+                (256, 1, 1, 1, 0, 6)):
+        with pytest.raises(E.EngineError, match="chunk_spins"):
+            b.set_schedule(*bad)
     check_run()                          # a rejected schedule leaves the batch as it was
     b.close()
 

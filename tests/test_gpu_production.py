@@ -212,6 +212,32 @@ def test_production_boundaries_match_the_reference(engine):
             assert edge(x[:, 0]) < 0.05 and edge(xo[:, 0]) < 0.05
 
 
+@pytest.mark.parametrize("dims", [(13, 7, 10), (8, 9, 12)])
+def test_production_gathers_every_voxel_exactly(engine, dims):
+    """With zero diffusivity nothing moves and no draw is taken, so the
+    production walk is deterministic: one spin at the centre of every voxel of a
+    phantom whose sides are not multiples of the record bricks (2 x 3 x 4 for the
+    production walk's float32 structure-of-arrays records, 4 x 2 x 2 for the
+    reference layout) must see that voxel's label and field, as the oracle does:
+    labels equal, M within 1e-5 (a GRE of 20 steps, each adding the voxel's
+    field as a phase)."""
+    m, f = cases.blocky_phantom(dims, n_labels=3, seed=7, block=2)
+    f = f * 30
+    cfg = cases.gre(3, TE=1000, dt=50, DIFFUSIVITY=[0.0, 0.0, 0.0], T1=[1000.0, 900.0, 800.0], T2=[60.0, 50.0, 40.0])
+    fov_um = 2.0 * np.asarray(dims, float)
+    idx = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), -1).reshape(-1, 3)
+    xyz0 = ((idx + 0.5) * 2e-6).astype(np.float32)
+    case = cases.make_case(cfg, m, f, fov_um, n_spins=len(xyz0), xyz0=xyz0)
+    for prec in (E.SW_PREC_F32, E.SW_PREC_F64):
+        g = run(engine, case, precision=prec)
+        o = orc.sim(cases.to_oracle(case), f64=prec == E.SW_PREC_F64)
+        assert g["stats"]["n_lost"] == 0 and g["stats"]["n_attempts"] == g["stats"]["n_steps"]
+        np.testing.assert_array_equal(g["T"][:, 0], m.reshape(-1))
+        np.testing.assert_array_equal(g["T"], o["T"])
+        mag = np.linalg.norm(o["M1"], axis=-1, keepdims=True)
+        assert (np.abs(g["M1"] - o["M1"]) <= 1e-5 * mag + 1e-7).all()
+
+
 def test_pgse_discrete_stejskal_tanner(engine):
     D, b, delta, DELTA, dt = 2e-9, 500.0, 5.0, 10.0, 50
     mask = np.zeros((8, 8, 8), np.uint8)
