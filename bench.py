@@ -271,11 +271,15 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
         if os.path.exists(tpath):
             with open(tpath) as fh:
                 t = json.load(fh)
-            # the line traffic depends on the schedule: report it only for the schedule it was measured under
-            if t.get("schedule") == res["schedule"]:
-                traffic, traffic_note = t.get("hbm_bytes_per_launch"), f"measured, {os.path.relpath(tpath, ROOT)}"
+            # The line traffic depends on the schedule: the file holds one measurement per schedule,
+            # and one is reported only for the schedule it was measured under.
+            runs = t.get("measurements", [t])
+            hit = [m for m in runs if m.get("schedule") == res["schedule"]]
+            if hit:
+                traffic, traffic_note = hit[0].get("hbm_bytes_per_launch"), f"measured, {os.path.relpath(tpath, ROOT)}"
             else:
-                traffic_note = f"{os.path.relpath(tpath, ROOT)} was measured under schedule {t.get('schedule')}"
+                traffic_note = (f"{os.path.relpath(tpath, ROOT)} was measured under schedules "
+                                f"{[m.get('schedule') for m in runs]}")
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
                 "traffic_rate": None if traffic is None else traffic / kernel_s / 1e9,

@@ -767,6 +767,8 @@ int fill_walk_args(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, con
         return fail(e, SW_E_INVALID, "mask has " + std::to_string(ph->n_labels) + " labels but only " +
                                          std::to_string(q->n_sub) + " substrates are configured");
     if (p->timestep_us <= 0) return fail(e, SW_E_INVALID, "timestep_us must be positive");
+    if (p->n_timepoints >= (1u << 31))            // the walk keeps a flag in the top bit of its step count
+        return fail(e, SW_E_INVALID, "n_timepoints must be below 2^31");
     if (p->n_dummy_scan < 0) return fail(e, SW_E_INVALID, "n_dummy_scan < 0: call sw_params_prepare first");
     if (!(p->fov[0] > 0 && p->fov[1] > 0 && p->fov[2] > 0)) return fail(e, SW_E_INVALID, "fov must be positive");
     if ((uint64_t)o->spin_offset + o->n_local > p->n_spins)

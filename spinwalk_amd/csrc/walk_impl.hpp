@@ -36,6 +36,7 @@ constexpr double kDeg2Rad = 0.0174532925199433;   // simulation_parameters.cuh:2
 constexpr double kRad2Deg = 57.2957795130823;     // simulation_parameters.cuh:25
 constexpr double kGamma   = 267515315.;           // definitions.h:20
 constexpr uint32_t kNever = 0xFFFFFFFFu;
+constexpr uint32_t kParked = 0x80000000u;   // set in a parked spin's time point (counts stay below 2^31)
 
 enum : int { ST_OK = 0, ST_LOST = 1, ST_EXHAUSTED = 2 };
 
@@ -147,7 +148,7 @@ struct RecF32 {                          // {float field_T, uint32 label}
     static constexpr BrickShape shape = kBrick8;
     static constexpr bool kSoA = false;
     using T = uint2;
-    __device__ static uint8_t label(T r) { return (uint8_t)r.y; }
+    __device__ static uint32_t label(T r) { return r.y; }
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
         if (sizeof(R) == 8) return (R)((double)__uint_as_float(r.x) * a.t2d_f64);
         return (R)(__uint_as_float(r.x) * a.t2d);
@@ -157,14 +158,14 @@ struct RecNone {                         // label only: {0, uint8 label, pad}
     static constexpr BrickShape shape = kBrick4;
     static constexpr bool kSoA = false;
     using T = uint32_t;
-    __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
+    __device__ static uint32_t label(T r) { return r >> 16; }
     template <class R> __device__ static R field(const WalkArgs &, T) { return (R)0; }
 };
 struct RecF16 {                          // {fp16 field_T/half_scale, uint8 label, pad}
     static constexpr BrickShape shape = kBrick4;
     static constexpr bool kSoA = false;
     using T = uint32_t;
-    __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
+    __device__ static uint32_t label(T r) { return r >> 16; }
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
         return (R)(__half2float(__ushort_as_half((unsigned short)(r & 0xFFFFu))) * a.h2d);
     }
@@ -173,7 +174,7 @@ struct RecQ16 {                          // {int16 rint(field_T/q_scale), uint8 
     static constexpr BrickShape shape = kBrick4;
     static constexpr bool kSoA = false;
     using T = uint32_t;
-    __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
+    __device__ static uint32_t label(T r) { return r >> 16; }
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
         return (R)((float)(int32_t)(int16_t)(uint16_t)(r & 0xFFFFu) * a.h2d);
     }
@@ -182,7 +183,7 @@ struct RecBF16 {                         // {bf16 field_T/half_scale, uint8 labe
     static constexpr BrickShape shape = kBrick4;
     static constexpr bool kSoA = false;
     using T = uint32_t;
-    __device__ static uint8_t label(T r) { return (uint8_t)(r >> 16); }
+    __device__ static uint32_t label(T r) { return r >> 16; }
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
         return (R)(__uint_as_float(r << 16) * a.h2d);
     }
@@ -196,13 +197,14 @@ struct RecF32S {
     static constexpr bool kSoA = true;
     static constexpr SoaShape soa = kSoaF32;
     using T = uint2;                     // {field bits, label}, as RecF32
-    __device__ static uint8_t label(T r) { return (uint8_t)r.y; }
+    __device__ static uint32_t label(T r) { return r.y; }
     template <class R> __device__ static R field(const WalkArgs &a, T r) { return RecF32::field<R>(a, r); }
     template <bool IDX32, class I> __device__ static T load(const void *base, I off) {
         const char *b = (const char *)base;
+        const I foff = off + (I)(soa.field_bytes - 1) * (off & (I)127);   // field - nvox, as an offset
         T r;
         r.y = *(const uint8_t *)(b + off);
-        r.x = *(const uint32_t *)(b + off + (I)soa.nvox + (I)(soa.field_bytes - 1) * (off & (I)127));
+        r.x = *(const uint32_t *)(b + soa.nvox + foff);
         return r;
     }
 };
@@ -211,6 +213,16 @@ template <class Rec, bool IDX32, class I>
 __device__ __forceinline__ typename Rec::T rec_load(const void *base, I off) {
     if constexpr (Rec::kSoA) return Rec::template load<IDX32>(base, off);
     else return load_rec<typename Rec::T, IDX32>(base, off);
+}
+
+// An empty statement that reads the next attempt's draws and claims to rewrite
+// the gathered record: the draw arithmetic stays between the gather's issue
+// and the first use of its result, hiding the load behind it.
+__device__ __forceinline__ void after_draws(uint32_t &r, float x, float y, float z) {
+    asm volatile("" : "+v"(r) : "v"(x), "v"(y), "v"(z));
+}
+__device__ __forceinline__ void after_draws(uint2 &r, float x, float y, float z) {
+    asm volatile("" : "+v"(r.x), "+v"(r.y) : "v"(x), "v"(y), "v"(z));
 }
 
 // ---------------------------------------------------------------------------
@@ -412,7 +424,7 @@ __device__ __forceinline__ void move(const WalkArgs &a, float px, float py, floa
     xn[2] = xo[2] + rw2;
     const bool out = (maybe_outside(xn[0], a.fov[0]) | maybe_outside(xn[1], a.fov[1]) |
                       maybe_outside(xn[2], a.fov[2])) != 0;
-    if (__builtin_expect(__ballot(out) != 0, 0)) {
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(out != 0) != 0, 0)) {
         xn[0] = fix_axis<CROSS>(a.fov[0], rw0, xn[0]);
         xn[1] = fix_axis<CROSS>(a.fov[1], rw1, xn[1]);
         xn[2] = fix_axis<CROSS>(a.fov[2], rw2, xn[2]);
@@ -432,7 +444,7 @@ struct PosRef {
     using T = double;
     struct Step {                          // metres per axis per step (parameters::prepare)
         double d = 0.;
-        __device__ void set(const WalkArgs &a, uint8_t ts) { d = a.diffusivity[ts]; }
+        __device__ void set(const WalkArgs &a, uint32_t ts) { d = a.diffusivity[ts]; }
         __device__ bool moves() const { return d != 0.; }          // kernels.cu:130
     };
     __device__ static T start(const WalkArgs &, float x0, int) { return (double)x0; }
@@ -444,6 +456,13 @@ struct PosRef {
     }
     template <class Rec, bool IDX32>
     __device__ static Vox<IDX32> find(const WalkArgs &a, const T x[3]) { return locate<Rec, IDX32>(a, x); }
+    // One attempt (kernels.cu:130-140): the step when the spin moves, then the lookup.
+    template <class Rec, bool CROSS, bool IDX32>
+    __device__ static Vox<IDX32> advance(const WalkArgs &a, bool moved, float px, float py, float pz,
+                                         const Step &s, const T xo[3], T xn[3]) {
+        if (moved) step<CROSS>(a, px, py, pz, s, xo, xn);
+        return find<Rec, IDX32>(a, xn);
+    }
     // kernels.cu:185: the gradient phase of a step, formed in double from float G
     template <class Phase>
     __device__ static void add_grad(Phase &ph, const WalkArgs &a, float Gx, float Gy, float Gz, const T x[3]) {
@@ -483,7 +502,7 @@ struct PosVox {
     using T = float;
     struct Step {                          // voxels per axis per step
         float d[3] = {0.f, 0.f, 0.f};
-        __device__ void set(const WalkArgs &a, uint8_t ts) {
+        __device__ void set(const WalkArgs &a, uint32_t ts) {
             const double D = a.diffusivity[ts];
             d[0] = (float)(D * a.s2g[0]);
             d[1] = (float)(D * a.s2g[1]);
@@ -501,7 +520,7 @@ struct PosVox {
         pn[2] = __builtin_fmaf(pz, s.d[2], po[2]);
         const bool out = (maybe_outside_f(pn[0], a.nf[0]) | maybe_outside_f(pn[1], a.nf[1]) |
                           maybe_outside_f(pn[2], a.nf[2])) != 0;
-        if (__builtin_expect(__ballot(out) != 0, 0)) {
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(out != 0) != 0, 0)) {
             pn[0] = fix_axis_vox<CROSS>(a.nf[0], px * s.d[0], pn[0]);
             pn[1] = fix_axis_vox<CROSS>(a.nf[1], py * s.d[1], pn[1]);
             pn[2] = fix_axis_vox<CROSS>(a.nf[2], pz * s.d[2], pn[2]);
@@ -517,6 +536,38 @@ struct PosVox {
             return v;
         }
         return locate_outside<Rec, IDX32>(a, (double)p[0], (double)p[1], (double)p[2]);
+    }
+    // One attempt: the step when the spin moves, then the lookup.  When every
+    // lane of the wave moves to a position whose coordinates all lie in [0, n)
+    // (+0 included; a float below the float nearest n truncates below n), the
+    // truncations are the subscripts and lie in range, so the record comes
+    // straight from the tables.  Otherwise the boundary correction and the
+    // checked lookup run, per lane as in step and find.  The test is a ballot,
+    // so the branch is uniform across the wave; the step is formed in every
+    // lane (its draws are finite) and a lane that does not move keeps po.
+    template <class Rec, bool CROSS, bool IDX32>
+    __device__ static Vox<IDX32> advance(const WalkArgs &a, bool moved, float px, float py, float pz,
+                                         const Step &s, const T po[3], T pn[3]) {
+        pn[0] = __builtin_fmaf(px, s.d[0], po[0]);
+        pn[1] = __builtin_fmaf(py, s.d[1], po[1]);
+        pn[2] = __builtin_fmaf(pz, s.d[2], po[2]);
+        const uint32_t out = (moved ? 0u : 1u) | maybe_outside_f(pn[0], a.nf[0]) | maybe_outside_f(pn[1], a.nf[1]) |
+                             maybe_outside_f(pn[2], a.nf[2]);
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(out != 0) == 0, 1)) {
+            Vox<IDX32> v;
+            v.ok = true;
+            v.rec = (RecIdx<IDX32>)record_of<Rec, IDX32>(a, cvt_i32_sat_f(pn[0]), cvt_i32_sat_f(pn[1]),
+                                                         cvt_i32_sat_f(pn[2]));
+            return v;
+        }
+        if (moved) {
+            pn[0] = fix_axis_vox<CROSS>(a.nf[0], px * s.d[0], pn[0]);
+            pn[1] = fix_axis_vox<CROSS>(a.nf[1], py * s.d[1], pn[1]);
+            pn[2] = fix_axis_vox<CROSS>(a.nf[2], pz * s.d[2], pn[2]);
+        } else {
+            pn[0] = po[0]; pn[1] = po[1]; pn[2] = po[2];           // a spin that does not move stays put
+        }
+        return find<Rec, IDX32>(a, pn);
     }
     // The gradient phase of a step from the position in metres: in double for the
     // FP64 walk, in float (g2d = 1e-3 dt 1e-6 gamma rad2deg) for the others.
@@ -574,7 +625,7 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
         if (!a.record_traj) for (int i = 0; i < 3; i++) xyz1[i] = P::metres(a, xo[i], i);
         return ST_LOST;
     }
-    uint8_t ts_old = Rec::label(rec_load<Rec, IDX32>(a.rec, vx.rec));   // :103
+    uint32_t ts_old = Rec::label(rec_load<Rec, IDX32>(a.rec, vx.rec));   // :103
     typename P::Step stp;
     stp.set(a, ts_old);                                            // :104
     // The reference re-reads T1/T2/diffusivity of ts_old at every voxel change
@@ -591,8 +642,10 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
     rng.peek();
     Phase ph;
     uint64_t slot = 0;
-    // Every early end (lost, stuck or out of draws) leaves through `done`, so the
-    // step loop has one exit edge besides its natural end.
+    // A spin that is lost or stuck parks: tp gains the kParked bit, which ends
+    // the step loop at its one exit, its natural end, and keeps the count of
+    // steps in the low bits; the TR loop then leaves through `done`.  Running
+    // out of replayed draws leaves through `done` at once.
     int st = ST_OK;
     uint32_t d = 0, tp = 0;
 
@@ -618,39 +671,49 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
 
         while (tp < a.n_timepoints) {                              // :128
             const bool moved = stp.moves();
-            if (moved) {                                           // :130-137
-                if (!rng.avail) { st = ST_EXHAUSTED; goto done; }
-                P::template step<CROSS>(a, rng.px, rng.py, rng.pz, stp, xo, xn);
-            }
-            vx = P::template find<Rec, IDX32>(a, xn);              // :140
-            if (!vx.ok) { st = ST_LOST; goto done; }               // :141-147
-            const bool newvox = vx.rec != rec_old;                 // :150
-            typename Rec::T r{};
-            if (newvox)                                            // one gather: label and field
-                r = rec_load<Rec, IDX32>(a.rec, vx.rec);
-            if (moved) {                                           // next attempt's draws while the load flies
-                rng.commit();
-                rng.peek();
-            }
-            if (newvox) {
-                const uint8_t ts = Rec::label(r);                  // :152
-                const R fnew = Rec::template field<R>(a, r);       // :166, read speculatively
-                if (ts != ts_old) {                                // :153-163
-                    float u;
-                    if (!rng.uniform(u)) { st = ST_EXHAUSTED; goto done; }
-                    if (u >= a.pXY[ts_old * a.n_sub + ts]) {
-                        ++n_rej;
-                        if (itr++ > a.max_iterations) { st = ST_LOST; goto done; }
-                        continue;
-                    }
-                    ts_old = ts;
-                    if (sizeof(R) == 8) { T1 = (R)a.T1d[ts]; T2 = (R)a.T2d[ts]; }
-                    else                { T1 = (R)a.T1s[ts]; T2 = (R)a.T2s[ts]; }
-                    stp.set(a, ts);
+            if (moved && !rng.avail) { st = ST_EXHAUSTED; goto done; }   // replayed draws used up
+            vx = P::template advance<Rec, CROSS, IDX32>(a, moved, rng.px, rng.py, rng.pz, stp, xo, xn);   // :130-140
+            bool acc = true;                                       // the attempt becomes a step
+            if (!vx.ok) {                                          // :141-147
+                tp |= kParked; acc = false;
+            } else {
+                const bool newvox = vx.rec != rec_old;             // :150
+                typename Rec::T r;
+                if (newvox)                                        // one gather: label and field
+                    r = rec_load<Rec, IDX32>(a.rec, vx.rec);
+                // The next attempt's draws while the load flies.  A spin that
+                // does not move never uses a draw (its voxel, hence its label,
+                // never changes), so the production generator commits in every
+                // lane; the replayed stream advances only on a move.
+                if (!Rng::kReplay || moved) {
+                    rng.commit();
+                    rng.peek();
                 }
-                rec_old = vx.rec;                                  // :165
-                field = fnew;
+                after_draws(r, rng.px, rng.py, rng.pz);
+                if (newvox) {
+                    const uint32_t ts = Rec::label(r);             // :152
+                    const R fnew = Rec::template field<R>(a, r);   // :166, read speculatively
+                    if (ts != ts_old) {                            // :153-163
+                        float u;
+                        if (!rng.uniform(u)) { st = ST_EXHAUSTED; goto done; }
+                        if (u >= a.pXY[ts_old * a.n_sub + ts]) {
+                            acc = false;
+                            ++n_rej;
+                            if (itr++ > a.max_iterations) tp |= kParked;
+                        } else {
+                            ts_old = ts;
+                            if (sizeof(R) == 8) { T1 = (R)a.T1d[ts]; T2 = (R)a.T2d[ts]; }
+                            else                { T1 = (R)a.T1s[ts]; T2 = (R)a.T2s[ts]; }
+                            stp.set(a, ts);
+                        }
+                    }
+                    if (acc) {
+                        rec_old = vx.rec;                          // :165
+                        field = fnew;
+                    }
+                }
             }
+            if (!acc) continue;
             ph.add(field);                                         // :171
             itr = 0;
 
@@ -688,7 +751,7 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
                     const uint64_t o = (uint64_t)li * a.n_te + te_i;
                     R *M1 = (R *)a.M1;
                     M1[3 * o + 0] = m[0]; M1[3 * o + 1] = m[1]; M1[3 * o + 2] = m[2];
-                    a.T[o] = ts_old;
+                    a.T[o] = (uint8_t)ts_old;
                     ph.reset();
                     old_tp = tp;
                     ++te_i;
@@ -704,6 +767,7 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
             xo[0] = xn[0]; xo[1] = xn[1]; xo[2] = xn[2];
             ++tp;
         }
+        if (tp & kParked) { st = ST_LOST; tp &= ~kParked; goto done; }
         const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);   // :226-231
         dephase_relax(m, ph.angle(), T1, T2, te);
     }
@@ -806,7 +870,7 @@ __device__ int walk_spin_views(const WalkArgs &a, const WalkView *xv, uint32_t l
                 for (int i = 0; i < 3; i++) xyz1_of(v)[i] = P::metres(a, xo[i], i);
         return ST_LOST;
     }
-    uint8_t ts_old = Rec::label(rec_load<Rec, IDX32>(a.rec, vx.rec));   // :103
+    uint32_t ts_old = Rec::label(rec_load<Rec, IDX32>(a.rec, vx.rec));   // :103
     typename P::Step stp;
     stp.set(a, ts_old);                                            // :104
     // The reference re-reads T1/T2/diffusivity of ts_old at every voxel change
@@ -823,8 +887,10 @@ __device__ int walk_spin_views(const WalkArgs &a, const WalkView *xv, uint32_t l
     rng.peek();
     Phase ph[NV];
     uint64_t slot = 0;
-    // Every early end (lost, stuck or out of draws) leaves through `done`, so the
-    // step loop has one exit edge besides its natural end.
+    // A spin that is lost or stuck parks: tp gains the kParked bit, which ends
+    // the step loop at its one exit, its natural end, and keeps the count of
+    // steps in the low bits; the TR loop then leaves through `done`.  Running
+    // out of replayed draws leaves through `done` at once.
     int st = ST_OK;
     uint32_t d = 0, tp = 0;
 
@@ -850,39 +916,49 @@ __device__ int walk_spin_views(const WalkArgs &a, const WalkView *xv, uint32_t l
 
         while (tp < a.n_timepoints) {                              // :128
             const bool moved = stp.moves();
-            if (moved) {                                           // :130-137
-                if (!rng.avail) { st = ST_EXHAUSTED; goto done; }
-                P::template step<CROSS>(a, rng.px, rng.py, rng.pz, stp, xo, xn);
-            }
-            vx = P::template find<Rec, IDX32>(a, xn);              // :140
-            if (!vx.ok) { st = ST_LOST; goto done; }               // :141-147
-            const bool newvox = vx.rec != rec_old;                 // :150
-            typename Rec::T r{};
-            if (newvox)                                            // one gather: label and field
-                r = rec_load<Rec, IDX32>(a.rec, vx.rec);
-            if (moved) {                                           // next attempt's draws while the load flies
-                rng.commit();
-                rng.peek();
-            }
-            if (newvox) {
-                const uint8_t ts = Rec::label(r);                  // :152
-                const R fnew = Rec::template field<R>(a, r);       // :166, read speculatively
-                if (ts != ts_old) {                                // :153-163
-                    float u;
-                    if (!rng.uniform(u)) { st = ST_EXHAUSTED; goto done; }
-                    if (u >= a.pXY[ts_old * a.n_sub + ts]) {
-                        ++n_rej;
-                        if (itr++ > a.max_iterations) { st = ST_LOST; goto done; }
-                        continue;
-                    }
-                    ts_old = ts;
-                    if (sizeof(R) == 8) { T1 = (R)a.T1d[ts]; T2 = (R)a.T2d[ts]; }
-                    else                { T1 = (R)a.T1s[ts]; T2 = (R)a.T2s[ts]; }
-                    stp.set(a, ts);
+            if (moved && !rng.avail) { st = ST_EXHAUSTED; goto done; }   // replayed draws used up
+            vx = P::template advance<Rec, CROSS, IDX32>(a, moved, rng.px, rng.py, rng.pz, stp, xo, xn);   // :130-140
+            bool acc = true;                                       // the attempt becomes a step
+            if (!vx.ok) {                                          // :141-147
+                tp |= kParked; acc = false;
+            } else {
+                const bool newvox = vx.rec != rec_old;             // :150
+                typename Rec::T r;
+                if (newvox)                                        // one gather: label and field
+                    r = rec_load<Rec, IDX32>(a.rec, vx.rec);
+                // The next attempt's draws while the load flies.  A spin that
+                // does not move never uses a draw (its voxel, hence its label,
+                // never changes), so the production generator commits in every
+                // lane; the replayed stream advances only on a move.
+                if (!Rng::kReplay || moved) {
+                    rng.commit();
+                    rng.peek();
                 }
-                rec_old = vx.rec;                                  // :165
-                field = fnew;
+                after_draws(r, rng.px, rng.py, rng.pz);
+                if (newvox) {
+                    const uint32_t ts = Rec::label(r);             // :152
+                    const R fnew = Rec::template field<R>(a, r);   // :166, read speculatively
+                    if (ts != ts_old) {                            // :153-163
+                        float u;
+                        if (!rng.uniform(u)) { st = ST_EXHAUSTED; goto done; }
+                        if (u >= a.pXY[ts_old * a.n_sub + ts]) {
+                            acc = false;
+                            ++n_rej;
+                            if (itr++ > a.max_iterations) tp |= kParked;
+                        } else {
+                            ts_old = ts;
+                            if (sizeof(R) == 8) { T1 = (R)a.T1d[ts]; T2 = (R)a.T2d[ts]; }
+                            else                { T1 = (R)a.T1s[ts]; T2 = (R)a.T2s[ts]; }
+                            stp.set(a, ts);
+                        }
+                    }
+                    if (acc) {
+                        rec_old = vx.rec;                          // :165
+                        field = fnew;
+                    }
+                }
             }
+            if (!acc) continue;
             for (int v = 0; v < NV; v++) ph[v].add(field);         // :171
             itr = 0;
 
@@ -929,7 +1005,7 @@ __device__ int walk_spin_views(const WalkArgs &a, const WalkView *xv, uint32_t l
                         dephase_relax(m[v], ph[v].angle(), T1, T2, te);
                         R *M1 = (R *)view_M1(a, xv, v);
                         M1[3 * o + 0] = m[v][0]; M1[3 * o + 1] = m[v][1]; M1[3 * o + 2] = m[v][2];
-                        view_T(a, xv, v)[o] = ts_old;
+                        view_T(a, xv, v)[o] = (uint8_t)ts_old;
                         ph[v].reset();
                     }
                     old_tp = tp;
@@ -949,6 +1025,7 @@ __device__ int walk_spin_views(const WalkArgs &a, const WalkView *xv, uint32_t l
             xo[0] = xn[0]; xo[1] = xn[1]; xo[2] = xn[2];
             ++tp;
         }
+        if (tp & kParked) { st = ST_LOST; tp &= ~kParked; goto done; }
         const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);   // :226-231
         for (int v = 0; v < NV; v++) dephase_relax(m[v], ph[v].angle(), T1, T2, te);
     }
