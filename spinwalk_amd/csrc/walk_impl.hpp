@@ -713,59 +713,60 @@ __device__ int walk_spin(const WalkArgs &a, uint32_t li, uint64_t &n_acc, uint32
                     }
                 }
             }
-            if (!acc) continue;
-            ph.add(field);                                         // :171
-            itr = 0;
+            if (acc) {
+                ph.add(field);                                         // :171
+                itr = 0;
 
-            if (tp == next_ev) {
-                if (tp == next_dp) {                               // :175-178
-                    if (sizeof(R) == 8)
-                        ph.add((R)((double)spin * a.deph_deg[dp_i] / (double)a.n_spins_total));
-                    else
-                        ph.add((R)((float)spin * a.deph_deg[dp_i] / (float)a.n_spins_total));
-                    ++dp_i;
-                    next_dp = (dp_i < a.n_deph) ? (uint32_t)a.deph_t[dp_i] : kNever;
+                if (tp == next_ev) {
+                    if (tp == next_dp) {                               // :175-178
+                        if (sizeof(R) == 8)
+                            ph.add((R)((double)spin * a.deph_deg[dp_i] / (double)a.n_spins_total));
+                        else
+                            ph.add((R)((float)spin * a.deph_deg[dp_i] / (float)a.n_spins_total));
+                        ++dp_i;
+                        next_dp = (dp_i < a.n_deph) ? (uint32_t)a.deph_t[dp_i] : kNever;
+                    }
+                    if (tp == next_gr) {                               // :181-187
+                        const float Gx = a.gx[gr_i] * a.gradient_scale;
+                        const float Gy = a.gy[gr_i] * a.gradient_scale;
+                        const float Gz = a.gz[gr_i] * a.gradient_scale;
+                        P::add_grad(ph, a, Gx, Gy, Gz, xn);
+                        ++gr_i;
+                        next_gr = (gr_i < a.n_grad) ? (uint32_t)a.grad_t[gr_i] : kNever;
+                    }
+                    if (tp == next_rf) {                               // :190-199
+                        const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
+                        dephase_relax(m, ph.angle(), T1, T2, te);
+                        const RfPulse p = a.rf[rf_i];
+                        if (sizeof(R) == 8) rot_apply<R>(p.kind_d, (R)p.sd, (R)p.cd, (R)p.spd, (R)p.cpd, m);
+                        else                rot_apply<R>(p.kind, (R)p.s, (R)p.c, (R)p.sp, (R)p.cp, m);
+                        ph.reset();
+                        old_tp = tp;
+                        ++rf_i;
+                        next_rf = (rf_i < a.n_rf) ? (uint32_t)a.rf[rf_i].t : kNever;
+                    }
+                    if (tp == next_te) {                               // :202-215
+                        const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
+                        dephase_relax(m, ph.angle(), T1, T2, te);
+                        const uint64_t o = (uint64_t)li * a.n_te + te_i;
+                        R *M1 = (R *)a.M1;
+                        M1[3 * o + 0] = m[0]; M1[3 * o + 1] = m[1]; M1[3 * o + 2] = m[2];
+                        a.T[o] = (uint8_t)ts_old;
+                        ph.reset();
+                        old_tp = tp;
+                        ++te_i;
+                        next_te = (te_i < a.n_te) ? (uint32_t)a.TE_t[te_i] : kNever;
+                    }
+                    next_ev = next_event(next_rf, next_te, next_dp, next_gr);
                 }
-                if (tp == next_gr) {                               // :181-187
-                    const float Gx = a.gx[gr_i] * a.gradient_scale;
-                    const float Gy = a.gy[gr_i] * a.gradient_scale;
-                    const float Gz = a.gz[gr_i] * a.gradient_scale;
-                    P::add_grad(ph, a, Gx, Gy, Gz, xn);
-                    ++gr_i;
-                    next_gr = (gr_i < a.n_grad) ? (uint32_t)a.grad_t[gr_i] : kNever;
+                if (a.record_traj) {                                   // :218-221
+                    float *dst = xyz1 + 3 * slot;
+                    dst[0] = P::metres(a, xn[0], 0); dst[1] = P::metres(a, xn[1], 1); dst[2] = P::metres(a, xn[2], 2);
+                    ++slot;
                 }
-                if (tp == next_rf) {                               // :190-199
-                    const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
-                    dephase_relax(m, ph.angle(), T1, T2, te);
-                    const RfPulse p = a.rf[rf_i];
-                    if (sizeof(R) == 8) rot_apply<R>(p.kind_d, (R)p.sd, (R)p.cd, (R)p.spd, (R)p.cpd, m);
-                    else                rot_apply<R>(p.kind, (R)p.s, (R)p.c, (R)p.sp, (R)p.cp, m);
-                    ph.reset();
-                    old_tp = tp;
-                    ++rf_i;
-                    next_rf = (rf_i < a.n_rf) ? (uint32_t)a.rf[rf_i].t : kNever;
-                }
-                if (tp == next_te) {                               // :202-215
-                    const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
-                    dephase_relax(m, ph.angle(), T1, T2, te);
-                    const uint64_t o = (uint64_t)li * a.n_te + te_i;
-                    R *M1 = (R *)a.M1;
-                    M1[3 * o + 0] = m[0]; M1[3 * o + 1] = m[1]; M1[3 * o + 2] = m[2];
-                    a.T[o] = (uint8_t)ts_old;
-                    ph.reset();
-                    old_tp = tp;
-                    ++te_i;
-                    next_te = (te_i < a.n_te) ? (uint32_t)a.TE_t[te_i] : kNever;
-                }
-                next_ev = next_event(next_rf, next_te, next_dp, next_gr);
+                xo[0] = xn[0]; xo[1] = xn[1]; xo[2] = xn[2];
+                ++tp;
             }
-            if (a.record_traj) {                                   // :218-221
-                float *dst = xyz1 + 3 * slot;
-                dst[0] = P::metres(a, xn[0], 0); dst[1] = P::metres(a, xn[1], 1); dst[2] = P::metres(a, xn[2], 2);
-                ++slot;
-            }
-            xo[0] = xn[0]; xo[1] = xn[1]; xo[2] = xn[2];
-            ++tp;
         }
         if (tp & kParked) { st = ST_LOST; tp &= ~kParked; goto done; }
         const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);   // :226-231
@@ -859,15 +860,20 @@ __device__ int walk_spin_views(const WalkArgs &a, const WalkView *xv, uint32_t l
     for (int i = 0; i < 3; i++) {                                  // kernels.cu:95-99
         const float x0 = a.XYZ0[3 * (uint64_t)li + i] * a.xyz_scale;
         xo[i] = xn[i] = P::start(a, x0, i);
+        #pragma unroll
         for (int v = 0; v < NV; v++) m[v][i] = (R)a.M0[3 * (uint64_t)li + i];
-        if (a.record_traj)
+        if (a.record_traj) {
+            #pragma unroll
             for (int v = 0; v < NV; v++) xyz1_of(v)[i] = x0;
+        }
     }
     Vox<IDX32> vx = P::template find<Rec, IDX32>(a, xo);           // kernels.cu:102
     if (!vx.ok) {                                                  // the reference reads out of bounds here
-        if (!a.record_traj)
+        if (!a.record_traj) {
+            #pragma unroll
             for (int v = 0; v < NV; v++)
                 for (int i = 0; i < 3; i++) xyz1_of(v)[i] = P::metres(a, xo[i], i);
+        }
         return ST_LOST;
     }
     uint32_t ts_old = Rec::label(rec_load<Rec, IDX32>(a.rec, vx.rec));   // :103
@@ -896,6 +902,7 @@ __device__ int walk_spin_views(const WalkArgs &a, const WalkView *xv, uint32_t l
 
     for (d = 0; d <= a.n_dummy; ++d) {                              // :107
         const bool last = (d == a.n_dummy);
+        #pragma unroll
         for (int v = 0; v < NV; v++) {                             // :110-120 excitation
             const Excitation e = view_exc(a, xv, v)[d];
             if (sizeof(R) == 8)
@@ -912,6 +919,7 @@ __device__ int walk_spin_views(const WalkArgs &a, const WalkView *xv, uint32_t l
         uint32_t next_dp = (a.n_deph > 0) ? (uint32_t)a.deph_t[0] : kNever;
         uint32_t next_gr = (a.n_grad > 0) ? (uint32_t)a.grad_t[0] : kNever;
         uint32_t next_ev = next_event(next_rf, next_te, next_dp, next_gr);
+        #pragma unroll
         for (int v = 0; v < NV; v++) ph[v].reset();
 
         while (tp < a.n_timepoints) {                              // :128
@@ -958,75 +966,83 @@ __device__ int walk_spin_views(const WalkArgs &a, const WalkView *xv, uint32_t l
                     }
                 }
             }
-            if (!acc) continue;
-            for (int v = 0; v < NV; v++) ph[v].add(field);         // :171
-            itr = 0;
+            if (acc) {
+                #pragma unroll
+                for (int v = 0; v < NV; v++) ph[v].add(field);         // :171
+                itr = 0;
 
-            if (tp == next_ev) {
-                if (tp == next_dp) {                               // :175-178
-                    for (int v = 0; v < NV; v++) {
-                        const float dd = view_deph_deg(a, xv, v)[dp_i];
-                        if (sizeof(R) == 8)
-                            ph[v].add((R)((double)spin * dd / (double)a.n_spins_total));
-                        else
-                            ph[v].add((R)((float)spin * dd / (float)a.n_spins_total));
+                if (tp == next_ev) {
+                    if (tp == next_dp) {                               // :175-178
+                        #pragma unroll
+                        for (int v = 0; v < NV; v++) {
+                            const float dd = view_deph_deg(a, xv, v)[dp_i];
+                            if (sizeof(R) == 8)
+                                ph[v].add((R)((double)spin * dd / (double)a.n_spins_total));
+                            else
+                                ph[v].add((R)((float)spin * dd / (float)a.n_spins_total));
+                        }
+                        ++dp_i;
+                        next_dp = (dp_i < a.n_deph) ? (uint32_t)a.deph_t[dp_i] : kNever;
                     }
-                    ++dp_i;
-                    next_dp = (dp_i < a.n_deph) ? (uint32_t)a.deph_t[dp_i] : kNever;
-                }
-                if (tp == next_gr) {                               // :181-187
-                    for (int v = 0; v < NV; v++) {
-                        const float gs = view_gradient_scale(a, xv, v);
-                        const float Gx = view_gx(a, xv, v)[gr_i] * gs;
-                        const float Gy = view_gy(a, xv, v)[gr_i] * gs;
-                        const float Gz = view_gz(a, xv, v)[gr_i] * gs;
-                        P::add_grad(ph[v], a, Gx, Gy, Gz, xn);
+                    if (tp == next_gr) {                               // :181-187
+                        #pragma unroll
+                        for (int v = 0; v < NV; v++) {
+                            const float gs = view_gradient_scale(a, xv, v);
+                            const float Gx = view_gx(a, xv, v)[gr_i] * gs;
+                            const float Gy = view_gy(a, xv, v)[gr_i] * gs;
+                            const float Gz = view_gz(a, xv, v)[gr_i] * gs;
+                            P::add_grad(ph[v], a, Gx, Gy, Gz, xn);
+                        }
+                        ++gr_i;
+                        next_gr = (gr_i < a.n_grad) ? (uint32_t)a.grad_t[gr_i] : kNever;
                     }
-                    ++gr_i;
-                    next_gr = (gr_i < a.n_grad) ? (uint32_t)a.grad_t[gr_i] : kNever;
-                }
-                if (tp == next_rf) {                               // :190-199
-                    const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
-                    for (int v = 0; v < NV; v++) {
-                        dephase_relax(m[v], ph[v].angle(), T1, T2, te);
-                        const RfPulse p = view_rf(a, xv, v)[rf_i];
-                        if (sizeof(R) == 8) rot_apply<R>(p.kind_d, (R)p.sd, (R)p.cd, (R)p.spd, (R)p.cpd, m[v]);
-                        else                rot_apply<R>(p.kind, (R)p.s, (R)p.c, (R)p.sp, (R)p.cp, m[v]);
-                        ph[v].reset();
+                    if (tp == next_rf) {                               // :190-199
+                        const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
+                        #pragma unroll
+                        for (int v = 0; v < NV; v++) {
+                            dephase_relax(m[v], ph[v].angle(), T1, T2, te);
+                            const RfPulse p = view_rf(a, xv, v)[rf_i];
+                            if (sizeof(R) == 8) rot_apply<R>(p.kind_d, (R)p.sd, (R)p.cd, (R)p.spd, (R)p.cpd, m[v]);
+                            else                rot_apply<R>(p.kind, (R)p.s, (R)p.c, (R)p.sp, (R)p.cp, m[v]);
+                            ph[v].reset();
+                        }
+                        old_tp = tp;
+                        ++rf_i;
+                        next_rf = (rf_i < a.n_rf) ? (uint32_t)a.rf[rf_i].t : kNever;
                     }
-                    old_tp = tp;
-                    ++rf_i;
-                    next_rf = (rf_i < a.n_rf) ? (uint32_t)a.rf[rf_i].t : kNever;
-                }
-                if (tp == next_te) {                               // :202-215
-                    const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
-                    const uint64_t o = (uint64_t)li * a.n_te + te_i;
-                    for (int v = 0; v < NV; v++) {
-                        dephase_relax(m[v], ph[v].angle(), T1, T2, te);
-                        R *M1 = (R *)view_M1(a, xv, v);
-                        M1[3 * o + 0] = m[v][0]; M1[3 * o + 1] = m[v][1]; M1[3 * o + 2] = m[v][2];
-                        view_T(a, xv, v)[o] = (uint8_t)ts_old;
-                        ph[v].reset();
+                    if (tp == next_te) {                               // :202-215
+                        const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
+                        const uint64_t o = (uint64_t)li * a.n_te + te_i;
+                        #pragma unroll
+                        for (int v = 0; v < NV; v++) {
+                            dephase_relax(m[v], ph[v].angle(), T1, T2, te);
+                            R *M1 = (R *)view_M1(a, xv, v);
+                            M1[3 * o + 0] = m[v][0]; M1[3 * o + 1] = m[v][1]; M1[3 * o + 2] = m[v][2];
+                            view_T(a, xv, v)[o] = (uint8_t)ts_old;
+                            ph[v].reset();
+                        }
+                        old_tp = tp;
+                        ++te_i;
+                        next_te = (te_i < a.n_te) ? (uint32_t)a.TE_t[te_i] : kNever;
                     }
-                    old_tp = tp;
-                    ++te_i;
-                    next_te = (te_i < a.n_te) ? (uint32_t)a.TE_t[te_i] : kNever;
+                    next_ev = next_event(next_rf, next_te, next_dp, next_gr);
                 }
-                next_ev = next_event(next_rf, next_te, next_dp, next_gr);
+                if (a.record_traj) {                                   // :218-221
+                    #pragma unroll
+                    for (int v = 0; v < NV; v++) {
+                        float *dst = xyz1_of(v) + 3 * slot;
+                        dst[0] = P::metres(a, xn[0], 0); dst[1] = P::metres(a, xn[1], 1);
+                        dst[2] = P::metres(a, xn[2], 2);
+                    }
+                    ++slot;
+                }
+                xo[0] = xn[0]; xo[1] = xn[1]; xo[2] = xn[2];
+                ++tp;
             }
-            if (a.record_traj) {                                   // :218-221
-                for (int v = 0; v < NV; v++) {
-                    float *dst = xyz1_of(v) + 3 * slot;
-                    dst[0] = P::metres(a, xn[0], 0); dst[1] = P::metres(a, xn[1], 1);
-                    dst[2] = P::metres(a, xn[2], 2);
-                }
-                ++slot;
-            }
-            xo[0] = xn[0]; xo[1] = xn[1]; xo[2] = xn[2];
-            ++tp;
         }
         if (tp & kParked) { st = ST_LOST; tp &= ~kParked; goto done; }
         const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);   // :226-231
+        #pragma unroll
         for (int v = 0; v < NV; v++) dephase_relax(m[v], ph[v].angle(), T1, T2, te);
     }
     d = a.n_dummy + 1;                                             // every TR done
@@ -1035,9 +1051,11 @@ done:
     // Without trajectories XYZ1 holds the last accepted position: the reference
     // stores every accepted position in place (kernels.cu:222-223), and a spin
     // that is lost keeps the one before.  Then the count of accepted steps.
-    if (!a.record_traj)
+    if (!a.record_traj) {
+        #pragma unroll
         for (int v = 0; v < NV; v++)
             for (int i = 0; i < 3; i++) xyz1_of(v)[i] = P::metres(a, xo[i], i);
+    }
     n_acc += (uint64_t)d * a.n_timepoints + tp;
     return st;
 }
