@@ -1449,9 +1449,9 @@ SW_API int sw_batch_set_schedule(sw_batch *b, const sw_batch_schedule *sc) {
 
 // Coordinate search over the schedule: starting from the current one, each
 // parameter in turn (chunk size, the band counts of the class kinds the batch
-// has, spins per lane, resident waves per SIMD, visiting-order brick) takes
-// the candidate value that times fastest, a candidate having to beat the best
-// by 1 %.  Each trial is one full run of the batch, timed with HIP events;
+// has, spins per lane, resident waves per SIMD, visiting-order brick, then the
+// chunk size and the waves once more) takes the candidate value that times
+// fastest, a candidate having to beat the best by 1 %.  Each trial is one full run of the batch, timed with HIP events;
 // schedules change only which spins run where and when, so every trial leaves
 // the batch's outputs as any run does.
 SW_API int sw_batch_tune(sw_batch *b, void *stream, uint32_t max_trials, sw_batch_schedule *best_out,
@@ -1492,10 +1492,21 @@ SW_API int sw_batch_tune(sw_batch *b, void *stream, uint32_t max_trials, sw_batc
         if (t < 0.99f * t_best) { best = cand; t_best = t; }
         return SW_OK;
     };
-    for (uint32_t c : {256u, 128u, 64u}) {
-        sw_batch_schedule x = best; x.chunk_spins = c;
-        if ((rc = consider(x))) return rc;
-    }
+    auto chunks = [&]() -> int {
+        for (uint32_t c : {256u, 128u, 64u}) {
+            sw_batch_schedule x = best; x.chunk_spins = c;
+            if (int r = consider(x)) return r;
+        }
+        return SW_OK;
+    };
+    auto caps = [&]() -> int {
+        for (uint32_t v : {5u, 4u, 3u, 0u}) {
+            sw_batch_schedule x = best; x.waves_per_simd = v;
+            if (int r = consider(x)) return r;
+        }
+        return SW_OK;
+    };
+    if ((rc = chunks())) return rc;
     if (kinds & 1u)
         for (uint32_t v : {1u << 20, 64u, 16u, 4u}) {
             sw_batch_schedule x = best; x.bands_shared = v;
@@ -1510,14 +1521,15 @@ SW_API int sw_batch_tune(sw_batch *b, void *stream, uint32_t max_trials, sw_batc
         sw_batch_schedule x = best; x.spins_per_lane = v;
         if ((rc = consider(x))) return rc;
     }
-    for (uint32_t v : {5u, 4u, 3u, 0u}) {
-        sw_batch_schedule x = best; x.waves_per_simd = v;
-        if ((rc = consider(x))) return rc;
-    }
+    if ((rc = caps())) return rc;
     for (uint32_t v : {16u, 4u, 32u, 8u}) {
         sw_batch_schedule x = best; x.order_brick = v;
         if ((rc = consider(x))) return rc;
     }
+    // The chunk size and the cap interact (a cap counts workgroups of the
+    // chunk's size): one more pass over both from the point reached.
+    if ((rc = chunks())) return rc;
+    if ((rc = caps())) return rc;
     rc = apply_schedule(e, b, best, s, true);
     if (rc) return rc;
     if (best_out) *best_out = best;

@@ -288,6 +288,37 @@ def test_impermeable_labels_never_change(engine):
         np.testing.assert_array_equal(g["T"][:, te], lab0)
 
 
+def test_still_tissue_beside_mobile_tissue(engine):
+    """A tissue of zero diffusivity beside a mobile one, spins spread over both
+    and walked in index order, so nearly every wave holds lanes that move and
+    lanes that do not (kernels.cu:130: no step without diffusivity).  A spin in
+    the still tissue keeps its start position (to the float rounding of the
+    voxel-unit conversion) and its label; a mobile spin walks, reflects off the
+    impermeable interface and never enters the still tissue."""
+    n_vox = 16
+    m = np.zeros((n_vox,) * 3, np.uint8)
+    m[: n_vox // 2] = 1                                           # x < fov/2: still tissue
+    cfg = cases.gre(2, TE=4000, dt=50, DIFFUSIVITY=[2e-9, 0.0], P_XY=[[1, 0], [0, 1]],
+                    T1=[1e9, 1e9], T2=[1e9, 1e9], MAX_ITERATIONS=1e6)
+    rng = np.random.default_rng(37)
+    n = 4096
+    fov_um = 32.0
+    xyz0 = rng.uniform(0.02, 0.98, size=(n, 3)) * fov_um * 1e-6
+    case = cases.make_case(cfg, m, None, fov_um, n_spins=n, xyz0=xyz0.astype(np.float32))
+    g = run(engine, case, spatial_order=0)
+    still = case.XYZ0[:, 0].astype(np.float64) < 0.5 * fov_um * 1e-6
+    assert 0.4 < still.mean() < 0.6 and g["stats"]["n_lost"] == 0
+    X0, X1 = case.XYZ0.astype(np.float64), g["XYZ1"][:, 0, :].astype(np.float64)
+    ulp = float(np.spacing(np.float32(fov_um * 1e-6)))
+    assert np.abs(X1[still] - X0[still]).max() <= 2 * ulp
+    np.testing.assert_array_equal(g["T"][still, 0], 1)
+    np.testing.assert_array_equal(g["T"][~still, 0], 0)
+    assert (X1[~still, 0] >= 0.5 * fov_um * 1e-6).all()
+    step = np.sqrt(2 * 2e-9 * 50e-6)                              # metres per axis per step
+    msd = ((X1[~still] - X0[~still]) ** 2).sum(axis=1).mean()
+    assert msd > 10 * step ** 2                                   # 80 steps: they walked
+
+
 def test_order_and_launch_split_invariance(engine):
     m, f = cases.three_substrate_phantom(n=24, fov_um=48.0, seed=36)
     cfg = cases.se(3, TE=6000, dt=50, P_XY=[[1, 0.5, 0.5], [0.5, 1, 0.5], [0.5, 0.5, 1]])
@@ -406,6 +437,10 @@ def test_walk_rejects_mismatched_buffers(engine):
     q = E.SwParams.from_buffer_copy(p)
     q.n_dummy_scan = -1
     with pytest.raises(E.EngineError, match="n_dummy_scan"):
+        call(params=q)
+    q = E.SwParams.from_buffer_copy(p)
+    q.n_timepoints = 1 << 31                                            # the walk flags lost spins in bit 31
+    with pytest.raises(E.EngineError, match="n_timepoints"):
         call(params=q)
     seq.close()
     ph.close()
