@@ -137,23 +137,27 @@ SW_HD float unit12(uint32_t r) {
 #endif
 }
 
-// The draws of one attempt from one Philox block r (Box-Muller):
-//   x = rho1 cos(2 pi f1), y = rho1 sin(2 pi f1), z = rho2 cos(2 pi f3),
-//   rho = sqrt(-2 ln(2 - f)) with f = unit12(r0) (rho1) or unit12(r2) (rho2),
+// The draws of one attempt from one Philox block r (Box-Muller), in units of
+// 1 / sqrt(2 ln 2), the factor the walk folds into its step sizes:
+//   x = s1 cos(2 pi f1), y = s1 sin(2 pi f1), z = s3 cos(2 pi f3),
+//   s = sqrt(-log2(2 - f)) with f = unit12(r0) (s1) or unit12(r2) (s3),
 //   f1 = unit12(r1), f3 = unit12(r3) (the angle in revolutions: cos(2 pi f) of
 //   f in [1, 2) is cos of its fraction), u = low_bytes3(r0, r1, r2) / 2^24.
-// On the device log2, sqrt, sin and cos are the hardware instructions
-// (v_log_f32 is log2; v_sin_f32 and v_cos_f32 take revolutions).
+// kDrawScale * s is the Box-Muller radius sqrt(-2 ln(2 - f)), so kDrawScale * x,
+// y and z are standard normals.  On the device log2, sqrt, sin and cos are the
+// hardware instructions (v_log_f32 is log2; v_sin_f32 and v_cos_f32 take
+// revolutions), and the negation rides on the square root's source modifier.
+constexpr double kDrawScale = 1.1774100225154747;              // sqrt(2 ln 2)
 #if defined(__HIPCC__)
 __device__ __forceinline__ void draw_bundle(const U32x4 &r, float &x, float &y, float &z, uint32_t &u) {
     const float l1 = __builtin_amdgcn_logf(2.0f - unit12(r.v[0]));
     const float l2 = __builtin_amdgcn_logf(2.0f - unit12(r.v[2]));
-    const float rho1 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * l1);    // -2 ln 2 log2(u)
-    const float rho2 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * l2);
+    const float s1 = __builtin_amdgcn_sqrtf(-l1);
+    const float s3 = __builtin_amdgcn_sqrtf(-l2);
     const float f1 = unit12(r.v[1]), f3 = unit12(r.v[3]);
-    x = rho1 * __builtin_amdgcn_cosf(f1);
-    y = rho1 * __builtin_amdgcn_sinf(f1);
-    z = rho2 * __builtin_amdgcn_cosf(f3);
+    x = s1 * __builtin_amdgcn_cosf(f1);
+    y = s1 * __builtin_amdgcn_sinf(f1);
+    z = s3 * __builtin_amdgcn_cosf(f3);
     u = low_bytes3(r.v[0], r.v[1], r.v[2]);
 }
 #endif

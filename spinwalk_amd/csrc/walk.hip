@@ -46,7 +46,8 @@ hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // The production draws, materialised (diagnostics and statistical tests).
-// out[(i*n_att + k)*4 + {0,1,2,3}] = {x, y, z, u} of attempt k of spin spin0+i.
+// out[(i*n_att + k)*4 + {0,1,2,3}] = {x, y, z, u} of attempt k of spin spin0+i,
+// the normals scaled to unit variance.
 // ---------------------------------------------------------------------------
 __global__ void philox_draws_kernel(uint64_t seed, uint32_t spin0, uint32_t n_spins, uint32_t n_att,
                                     float *out) {
@@ -58,7 +59,9 @@ __global__ void philox_draws_kernel(uint64_t seed, uint32_t spin0, uint32_t n_sp
     PhiloxRng r(a, 0, spin0 + i);
     r.attempt = k;
     r.peek();
-    out[4 * g + 0] = r.px; out[4 * g + 1] = r.py; out[4 * g + 2] = r.pz; out[4 * g + 3] = u24_to_unit(r.pu);
+    const float c = (float)kDrawScale;                              // the bundle's unit: standard normals
+    out[4 * g + 0] = r.px * c; out[4 * g + 1] = r.py * c; out[4 * g + 2] = r.pz * c;
+    out[4 * g + 3] = u24_to_unit(r.pu);
 }
 
 hipError_t launch_philox_draws(uint64_t seed, uint32_t spin0, uint32_t n_spins, uint32_t n_att,

@@ -500,32 +500,23 @@ __device__ __forceinline__ float fix_axis_vox(float n, float rw, float v) {
 
 struct PosVox {
     using T = float;
-    struct Step {                          // voxels per axis per step
-        float d[3] = {0.f, 0.f, 0.f};
+    // Voxels per axis per step, times kDrawScale: the production draws come in
+    // units of 1 / kDrawScale (draw_bundle).  A tissue that does not move (zero
+    // diffusivity, kernels.cu:130) keeps NaN in d[0]: the step formed from it
+    // is NaN, which the in-FoV test reads as outside, so such a lane takes the
+    // checked path and stays where it is.
+    struct Step {
+        float d[3] = {__builtin_nanf(""), 0.f, 0.f};
         __device__ void set(const WalkArgs &a, uint32_t ts) {
-            const double D = a.diffusivity[ts];
-            d[0] = (float)(D * a.s2g[0]);
+            const double D = a.diffusivity[ts] * kDrawScale;
+            d[0] = D != 0. ? (float)(D * a.s2g[0]) : __builtin_nanf("");
             d[1] = (float)(D * a.s2g[1]);
             d[2] = (float)(D * a.s2g[2]);
         }
-        __device__ bool moves() const { return d[0] != 0.f; }
+        __device__ bool moves() const { return d[0] == d[0]; }
     };
     __device__ static T start(const WalkArgs &a, float x0, int i) { return x0 * a.s2gf[i]; }
     __device__ static float metres(const WalkArgs &a, T p, int i) { return p * a.vsf[i]; }
-    template <bool CROSS>
-    __device__ static void step(const WalkArgs &a, float px, float py, float pz, const Step &s, const T po[3],
-                                T pn[3]) {
-        pn[0] = __builtin_fmaf(px, s.d[0], po[0]);
-        pn[1] = __builtin_fmaf(py, s.d[1], po[1]);
-        pn[2] = __builtin_fmaf(pz, s.d[2], po[2]);
-        const bool out = (maybe_outside_f(pn[0], a.nf[0]) | maybe_outside_f(pn[1], a.nf[1]) |
-                          maybe_outside_f(pn[2], a.nf[2])) != 0;
-        if (__builtin_expect(__builtin_amdgcn_ballot_w64(out != 0) != 0, 0)) {
-            pn[0] = fix_axis_vox<CROSS>(a.nf[0], px * s.d[0], pn[0]);
-            pn[1] = fix_axis_vox<CROSS>(a.nf[1], py * s.d[1], pn[1]);
-            pn[2] = fix_axis_vox<CROSS>(a.nf[2], pz * s.d[2], pn[2]);
-        }
-    }
     template <class Rec, bool IDX32>
     __device__ static Vox<IDX32> find(const WalkArgs &a, const T p[3]) {
         const uint32_t ix = cvt_i32_sat_f(p[0]), iy = cvt_i32_sat_f(p[1]), iz = cvt_i32_sat_f(p[2]);
@@ -544,15 +535,15 @@ struct PosVox {
     // straight from the tables.  Otherwise the boundary correction and the
     // checked lookup run, per lane as in step and find.  The test is a ballot,
     // so the branch is uniform across the wave; the step is formed in every
-    // lane (its draws are finite) and a lane that does not move keeps po.
+    // lane, and a lane that does not move (a NaN step) keeps po.
     template <class Rec, bool CROSS, bool IDX32>
     __device__ static Vox<IDX32> advance(const WalkArgs &a, bool moved, float px, float py, float pz,
                                          const Step &s, const T po[3], T pn[3]) {
         pn[0] = __builtin_fmaf(px, s.d[0], po[0]);
         pn[1] = __builtin_fmaf(py, s.d[1], po[1]);
         pn[2] = __builtin_fmaf(pz, s.d[2], po[2]);
-        const uint32_t out = (moved ? 0u : 1u) | maybe_outside_f(pn[0], a.nf[0]) | maybe_outside_f(pn[1], a.nf[1]) |
-                             maybe_outside_f(pn[2], a.nf[2]);
+        const uint32_t out = maybe_outside_f(pn[0], a.nf[0]) | maybe_outside_f(pn[1], a.nf[1]) |
+                             maybe_outside_f(pn[2], a.nf[2]);      // a NaN step is outside
         if (__builtin_expect(__builtin_amdgcn_ballot_w64(out != 0) == 0, 1)) {
             Vox<IDX32> v;
             v.ok = true;
