@@ -816,6 +816,11 @@ int fill_walk_args(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, con
     a.g2d = (float)(1e-3 * p->timestep_us * 1e-6 * kGamma * kRad2Deg);
     a.diffusivity = q->diffusivity; a.pXY = q->pXY; a.T1s = q->T1s; a.T2s = q->T2s; a.T1d = q->T1d; a.T2d = q->T2d;
     a.n_sub = q->n_sub;
+    // The tissue tables go to LDS beside the per-axis tables of the 32-bit mode
+    // when both fit its 64 KiB (a walk in the 64-bit mode ignores the flag).
+    a.tissue_lds = (o->rng != SW_RNG_REPLAY && q->n_sub <= sw::kTissueLdsMax &&
+                    ((uint64_t)p->phantom_size[0] + p->phantom_size[1] + p->phantom_size[2] +
+                     (uint64_t)q->n_sub * q->n_sub + 5ull * q->n_sub) * 4 <= 65536) ? 1u : 0u;
     a.rf = q->rf; a.n_rf = q->n_rf;
     a.TE_t = q->TE_t; a.n_te = q->n_te;
     a.deph_t = q->deph_t; a.deph_deg = q->deph_deg; a.n_deph = q->n_deph;
@@ -965,7 +970,7 @@ struct sw_batch {
     std::vector<Reduce> reduces;
     int rng = 0, field = 0, prec = 0;
     bool idx32 = false;
-    size_t lds = 0;
+    size_t lds = 0;                              // LDS of a workgroup: per-axis tables, then tissue tables (32-bit mode)
     // what the chunk table is built from: per walk unit (a single tile, or a
     // fused group of tiles), its CROSS_FOV group, spins, steps per spin, phantom
     // records, views (1: d_items[index]; kMaxViews: d_fused[index])

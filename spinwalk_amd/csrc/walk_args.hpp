@@ -104,6 +104,7 @@ struct WalkArgs {
     const float    *T1s, *T2s;    // [n_sub] seconds, (float)(ms * 1e-3) (kernels.cu:167-168)
     const double   *T1d, *T2d;    // [n_sub] seconds in double
     uint32_t        n_sub;
+    uint32_t        tissue_lds;   // 1: each workgroup copies the tissue tables to LDS (tissue_lds_words)
     const RfPulse  *rf;           // [n_rf]; entry 0 unused (excitation)
     uint32_t        n_rf;
     const int32_t  *TE_t;         uint32_t n_te;
@@ -137,6 +138,14 @@ struct WalkArgs {
     // ---- counters: [0] lost, [1] replay exhausted, [2] accepted steps, [3] attempts
     unsigned long long *counters;
 };
+
+// Production walks in the 32-bit mode with at most kTissueLdsMax tissues keep
+// their per-tissue tables in LDS, after the per-axis tables: n_sub^2 crossing
+// probabilities, T1, T2 and the three step sizes of each tissue, 32-bit words.
+constexpr uint32_t kTissueLdsMax = 16;
+SW_HD_INLINE uint32_t tissue_lds_words(const WalkArgs &a) {
+    return a.tissue_lds ? a.n_sub * a.n_sub + 5 * a.n_sub : 0u;
+}
 
 // An item of the fused batched walk: view 0 is `a`, views 1..n_views-1 are xv;
 // entries of xv beyond them repeat view 0 with counters = NULL, so the kernel
