@@ -138,10 +138,26 @@ struct PhaseF64 {
 // in float once on the host (monte_carlo.cu:243-244); multiplying the stored
 // field_T by the same float factor at the gather gives the same float.
 // ---------------------------------------------------------------------------
+// Loads through global (not flat) pointers: a flat load also counts against the
+// LDS/scalar-load counter, so waiting on an LDS read or a scalar load would wait
+// on the gather too; and with a 32-bit offset the load takes the base from
+// scalar registers and the offset from one vector register.
+template <class T> using GlobalPtr = const __attribute__((address_space(1))) T *;
+template <class T, class I>
+__device__ __forceinline__ T gload(const void *base, I byte_off) {
+    const GlobalPtr<char> p = (GlobalPtr<char>)base + byte_off;
+    if constexpr (std::is_same<T, uint2>::value) {
+        const uint64_t v = *(GlobalPtr<uint64_t>)p;
+        return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+    } else {
+        return *(GlobalPtr<T>)p;
+    }
+}
+
 template <class T, bool IDX32>
 __device__ __forceinline__ T load_rec(const void *base, uint64_t i) {
-    if (IDX32) return *(const T *)((const char *)base + (uint32_t)((uint32_t)i * (uint32_t)sizeof(T)));
-    return ((const T *)base)[i];
+    if (IDX32) return gload<T>(base, (uint32_t)((uint32_t)i * (uint32_t)sizeof(T)));
+    return gload<T>(base, i * sizeof(T));
 }
 
 struct RecF32 {                          // {float field_T, uint32 label}
@@ -200,11 +216,10 @@ struct RecF32S {
     __device__ static uint32_t label(T r) { return r.y; }
     template <class R> __device__ static R field(const WalkArgs &a, T r) { return RecF32::field<R>(a, r); }
     template <bool IDX32, class I> __device__ static T load(const void *base, I off) {
-        const char *b = (const char *)base;
         const I foff = off + (I)(soa.field_bytes - 1) * (off & (I)127);   // field - nvox, as an offset
         T r;
-        r.y = *(const uint8_t *)(b + off);
-        r.x = *(const uint32_t *)(b + soa.nvox + foff);
+        r.y = gload<uint8_t>(base, off);
+        r.x = gload<uint32_t>((const char *)base + soa.nvox, foff);
         return r;
     }
 };
@@ -508,7 +523,7 @@ struct PosVox {
     struct Step {
         float d[3] = {__builtin_nanf(""), 0.f, 0.f};
         __device__ void set(const WalkArgs &a, uint32_t ts) {
-            const double D = a.diffusivity[ts] * kDrawScale;
+            const double D = gload<double>(a.diffusivity, (uint64_t)ts * 8) * kDrawScale;
             d[0] = D != 0. ? (float)(D * a.s2g[0]) : __builtin_nanf("");
             d[1] = (float)(D * a.s2g[1]);
             d[2] = (float)(D * a.s2g[2]);
@@ -612,7 +627,7 @@ struct Tissue {
     __device__ float pxy(uint32_t from, uint32_t to) const {                 // kernels.cu:155
         const uint32_t i = from * a.n_sub + to;
         if (kLds && a.tissue_lds) return tab()[i];
-        const float v = a.pXY[i];
+        const float v = gload<float>(a.pXY, (uint64_t)i * 4);
         asm volatile("" ::);
         return v;
     }
@@ -624,7 +639,7 @@ struct Tissue {
                 const float *t = tab() + a.n_sub * a.n_sub;
                 T1 = t[ts]; T2 = t[a.n_sub + ts];
             } else {
-                T1 = (R)a.T1s[ts]; T2 = (R)a.T2s[ts];
+                T1 = (R)gload<float>(a.T1s, (uint64_t)ts * 4); T2 = (R)gload<float>(a.T2s, (uint64_t)ts * 4);
                 asm volatile("" ::);
             }
         }
