@@ -54,6 +54,15 @@ extern "C" {
 #define SW_FIELD_Q16        4   /* int16 fixed point x per-phantom scale (max|field| */
                                 /* is 32767 steps); absolute error <= max|field|/65534 */
 
+/* The step loop of a single production walk (sw_walk_opts.step_loop): one
+ * attempt per trip, or two (the second formed from the first's position and
+ * both gathers in flight together; the same draws, steps and results).  Auto
+ * takes two when the gathered records exceed 256 MiB, the Infinity Cache of an
+ * MI355X.  Replayed walks and batches take one. */
+#define SW_STEP_LOOP_AUTO   0
+#define SW_STEP_LOOP_ONE    1
+#define SW_STEP_LOOP_TWO    2
+
 #define SW_DTYPE_F32        0
 #define SW_DTYPE_F64        1
 
@@ -102,6 +111,7 @@ typedef struct sw_walk_opts {
     int32_t  spatial_order;    /* 1 = visit spins in start-voxel brick order           */
     const float *replay_normals;  uint64_t replay_normal_stride;   /* device, [n_local][stride] */
     const float *replay_uniforms; uint64_t replay_uniform_stride;  /* device, [n_local][stride] */
+    int32_t  step_loop;        /* SW_STEP_LOOP_*; 0 (auto) when zero-initialised       */
 } sw_walk_opts;
 
 /* Per-launch counters returned by the walk. */

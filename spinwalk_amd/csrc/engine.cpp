@@ -30,6 +30,7 @@ namespace {
 constexpr double kDeg2Rad = 0.0174532925199433;   // simulation_parameters.cuh:24
 constexpr double kRad2Deg = 57.2957795130823;     // simulation_parameters.cuh:25
 constexpr double kGamma   = 267515315.;           // definitions.h:20
+constexpr uint64_t kPairsMinBytes = 256ull << 20;   // records beyond the Infinity Cache: two attempts per trip
 
 thread_local std::string g_tls_error;
 
@@ -844,6 +845,17 @@ int fill_walk_args(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, con
     a.M0 = d_M0; a.XYZ0 = d_XYZ0; a.M1 = d_M1; a.XYZ1 = d_XYZ1; a.T = d_T;
     a.rn = o->replay_normals; a.rn_stride = o->replay_normal_stride;
     a.ru = o->replay_uniforms; a.ru_stride = o->replay_uniform_stride;
+    // The step loop of a single production walk: two attempts per trip when the
+    // records it gathers exceed the 256 MiB Infinity Cache, where each gather
+    // waits on HBM and a second one in flight pays (both loops give the same
+    // results).  Replayed walks and batches take one attempt per trip.
+    if (o->step_loop < SW_STEP_LOOP_AUTO || o->step_loop > SW_STEP_LOOP_TWO)
+        return fail(e, SW_E_INVALID, "unknown step_loop");
+    a.pairs = 0;
+    if (o->rng == SW_RNG_PHILOX) {
+        if (o->step_loop == SW_STEP_LOOP_TWO) a.pairs = 1;
+        else if (o->step_loop == SW_STEP_LOOP_AUTO) a.pairs = sw::walk_mode(a, field, 0).bytes > kPairsMinBytes ? 1u : 0u;
+    }
     return SW_OK;
 }
 

@@ -105,6 +105,7 @@ struct WalkArgs {
     const double   *T1d, *T2d;    // [n_sub] seconds in double
     uint32_t        n_sub;
     uint32_t        tissue_lds;   // 1: each workgroup copies the tissue tables to LDS (tissue_lds_words)
+    uint32_t        pairs;        // 1: a single production walk takes two attempts per trip of its step loop
     const RfPulse  *rf;           // [n_rf]; entry 0 unused (excitation)
     uint32_t        n_rf;
     const int32_t  *TE_t;         uint32_t n_te;

@@ -289,6 +289,14 @@ struct PhiloxRng {          // production: Philox4x32-10, one block per attempte
     __device__ void commit() { ucur = pu; ++attempt; }
     // the permeability uniform of the attempt last committed
     __device__ bool uniform(float &v) { v = u24_to_unit(ucur); return true; }
+    // The draws of attempt k (the bundle peek() makes when attempt == k).
+    struct Draw { float x, y, z; uint32_t u; };
+    __device__ Draw draw(uint64_t k) const {
+        const U32x4 r = philox4x32_10_pre((uint32_t)k, (uint32_t)(k >> 32), m_hi, m_lo, 0u, k0, k1);
+        Draw q;
+        draw_bundle(r, q.x, q.y, q.z, q.u);
+        return q;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -857,7 +865,11 @@ done:
 // One workgroup walks the spins at positions [t0, t0 + spl * blockDim.x) of the
 // launch's visiting order (fewer at the end): lane l takes positions
 // t0 + k * blockDim.x + l, k = 0 .. spl-1, one after the other.
-template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
+template <class Rec, class Phase, bool CROSS, bool IDX32, int NV>
+__device__ int walk_spin_pairs(const WalkArgs &a, const WalkView *xv, uint32_t li, uint64_t &n_acc, uint32_t &n_rej);
+
+// PAIRS: the production step loop with two attempts per trip (walk_spin_pairs).
+template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32, bool PAIRS = false>
 __device__ __forceinline__ void walk_block(const WalkArgs &a, uint32_t t0, uint32_t spl = 1) {
     if (IDX32) {
         const uint32_t n_tab = (uint32_t)(a.nx + a.ny + a.nz);
@@ -871,7 +883,9 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, uint32_t t0, uint3
         const uint32_t t = t0 + k * blockDim.x + threadIdx.x;
         if (t < a.n_local) {
             const uint32_t li = a.order ? a.order[t] : t;
-            const int st = walk_spin<Rng, Rec, Phase, CROSS, IDX32>(a, li, n_acc, n_rej);
+            int st;
+            if constexpr (Rng::kReplay || !PAIRS) st = walk_spin<Rng, Rec, Phase, CROSS, IDX32>(a, li, n_acc, n_rej);
+            else st = walk_spin_pairs<Rec, Phase, CROSS, IDX32, 1>(a, nullptr, li, n_acc, n_rej);
             lost += (st == ST_LOST);
             exh += (st == ST_EXHAUSTED);
         }
@@ -1131,6 +1145,239 @@ done:
     return st;
 }
 
+// ---------------------------------------------------------------------------
+// The production walk (Philox draws, float voxel-unit positions), NV views:
+// walk_spin_views with two attempts per trip of the step loop.  Attempt A moves
+// from the current position with its draws; attempt B, the one after it, is
+// formed at the same time from A's position with the next draws, and both
+// gathers are issued together, so that a wave waits once for two.  B stands when
+// A becomes a step, the spin keeps its tissue (hence its step sizes) and the TR
+// has a time point left; otherwise B is dropped and made again in the next trip
+// from wherever A left the spin, with the same draws.  The draws of attempt k
+// are the k-th bundle of the spin whichever trip makes it, so every attempt,
+// step, rejection and event happens as in walk_spin_views, bit for bit.
+// Both gathers are issued by every lane, a lane that needs none reading record
+// 0, so that the loads are straight-line code and the wait for them is one.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void after_draws(T &ra, T &rb, const PhiloxRng::Draw &q2, const PhiloxRng::Draw &q3) {
+    after_draws(ra, q2.x, q2.y, q2.z);
+    after_draws(rb, q3.x, q3.y, q3.z);
+}
+
+template <class Rec, class Phase, bool CROSS, bool IDX32, int NV>
+__device__ int walk_spin_pairs(const WalkArgs &a, const WalkView *xv, uint32_t li, uint64_t &n_acc,
+                               uint32_t &n_rej) {
+    using Rng = PhiloxRng;
+    using R = typename Phase::R;
+    using P = PosVox;
+    using Draw = PhiloxRng::Draw;
+    const uint32_t spin = a.spin_offset + li;
+    const uint64_t xoff = 3 * (uint64_t)li * a.traj_len;
+    float *const xyz1_0 = a.XYZ1 + xoff;
+    auto xyz1_of = [&](int v) -> float * { return v == 0 ? xyz1_0 : view_XYZ1(a, xv, v) + xoff; };
+    float xo[3], xa[3], xb[3];
+    R m[NV][3];
+    for (int i = 0; i < 3; i++) {                                  // kernels.cu:95-99
+        const float x0 = a.XYZ0[3 * (uint64_t)li + i] * a.xyz_scale;
+        xo[i] = xa[i] = xb[i] = P::start(a, x0, i);
+        #pragma unroll
+        for (int v = 0; v < NV; v++) m[v][i] = (R)a.M0[3 * (uint64_t)li + i];
+        if (a.record_traj) {
+            #pragma unroll
+            for (int v = 0; v < NV; v++) xyz1_of(v)[i] = x0;
+        }
+    }
+    const Vox<IDX32> v0 = P::template find<Rec, IDX32>(a, xo);     // kernels.cu:102
+    if (!v0.ok) {
+        if (!a.record_traj) {
+            #pragma unroll
+            for (int v = 0; v < NV; v++)
+                for (int i = 0; i < 3; i++) xyz1_of(v)[i] = P::metres(a, xo[i], i);
+        }
+        return ST_LOST;
+    }
+    uint32_t ts_old = Rec::label(rec_load<Rec, IDX32>(a.rec, v0.rec));   // :103
+    const Tissue<Rng, IDX32> tis{a};
+    typename P::Step stp;
+    tis.step(ts_old, stp);                                         // :104
+    R field = 0, T1 = 0, T2 = 0;
+    if (a.n_timepoints > 0) {
+        tis.relax(ts_old, T1, T2);
+    }
+    uint32_t itr = 0;
+    const Rng rng(a, li, spin);
+    uint64_t att = 0;                                              // attempt of q0
+    Draw q0 = rng.draw(0), q1 = rng.draw(1);
+    Phase ph[NV];
+    uint64_t slot = 0;
+    int st = ST_OK;
+    uint32_t d = 0, tp = 0;
+
+    for (d = 0; d <= a.n_dummy; ++d) {                              // :107
+        const bool last = (d == a.n_dummy);
+        #pragma unroll
+        for (int v = 0; v < NV; v++) {                             // :110-120 excitation
+            const Excitation e = view_exc(a, xv, v)[d];
+            if (sizeof(R) == 8)
+                rot_apply<R>(e.kind_d, (R)view_exc_sd(a, xv, v), (R)view_exc_cd(a, xv, v), (R)e.spd, (R)e.cpd, m[v]);
+            else
+                rot_apply<R>(e.kind, (R)view_exc_s(a, xv, v), (R)view_exc_c(a, xv, v), (R)e.sp, (R)e.cp, m[v]);
+        }
+        RecIdx<IDX32> rec_old = ~(RecIdx<IDX32>)0;                 // :123-126
+        uint32_t old_tp = 0;
+        tp = 0;
+        uint32_t rf_i = 1, te_i = 0, dp_i = 0, gr_i = 0;
+        uint32_t next_rf = (a.n_rf > 1) ? (uint32_t)a.rf[1].t : kNever;
+        uint32_t next_te = (last && a.n_te > 0) ? (uint32_t)a.TE_t[0] : kNever;
+        uint32_t next_dp = (a.n_deph > 0) ? (uint32_t)a.deph_t[0] : kNever;
+        uint32_t next_gr = (a.n_grad > 0) ? (uint32_t)a.grad_t[0] : kNever;
+        uint32_t next_ev = next_event(next_rf, next_te, next_dp, next_gr);
+        #pragma unroll
+        for (int v = 0; v < NV; v++) ph[v].reset();
+
+        // One attempt after its gather: kernels.cu:141-221 as in walk_spin_views.
+        // Returns whether it became a step; *kept is cleared when the spin
+        // changed tissue.
+        auto attempt = [&](const Vox<IDX32> &vx, const typename Rec::T &r, bool newvox, const float *xn,
+                           uint32_t ubits, bool &kept) -> bool {
+            bool acc = true;
+            if (!vx.ok) {                                          // :141-147
+                tp |= kParked;
+                return false;
+            }
+            if (newvox) {
+                const uint32_t ts = Rec::label(r);                 // :152
+                const R fnew = Rec::template field<R>(a, r);       // :166
+                if (ts != ts_old) {                                // :153-163
+                    if (u24_to_unit(ubits) >= tis.pxy(ts_old, ts)) {
+                        acc = false;
+                        ++n_rej;
+                        if (itr++ > a.max_iterations) tp |= kParked;
+                    } else {
+                        ts_old = ts;
+                        tis.relax(ts, T1, T2);
+                        tis.step(ts, stp);
+                        kept = false;
+                    }
+                }
+                if (acc) {
+                    rec_old = vx.rec;                              // :165
+                    field = fnew;
+                }
+            }
+            if (acc) {
+                #pragma unroll
+                for (int v = 0; v < NV; v++) ph[v].add(field);     // :171
+                itr = 0;
+                if (tp == next_ev) {
+                    if (tp == next_dp) {                           // :175-178
+                        #pragma unroll
+                        for (int v = 0; v < NV; v++) {
+                            const float dd = view_deph_deg(a, xv, v)[dp_i];
+                            if (sizeof(R) == 8)
+                                ph[v].add((R)((double)spin * dd / (double)a.n_spins_total));
+                            else
+                                ph[v].add((R)((float)spin * dd / (float)a.n_spins_total));
+                        }
+                        ++dp_i;
+                        next_dp = (dp_i < a.n_deph) ? (uint32_t)a.deph_t[dp_i] : kNever;
+                    }
+                    if (tp == next_gr) {                           // :181-187
+                        #pragma unroll
+                        for (int v = 0; v < NV; v++) {
+                            const float gs = view_gradient_scale(a, xv, v);
+                            const float Gx = view_gx(a, xv, v)[gr_i] * gs;
+                            const float Gy = view_gy(a, xv, v)[gr_i] * gs;
+                            const float Gz = view_gz(a, xv, v)[gr_i] * gs;
+                            P::add_grad(ph[v], a, Gx, Gy, Gz, xn);
+                        }
+                        ++gr_i;
+                        next_gr = (gr_i < a.n_grad) ? (uint32_t)a.grad_t[gr_i] : kNever;
+                    }
+                    if (tp == next_rf) {                           // :190-199
+                        const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
+                        #pragma unroll
+                        for (int v = 0; v < NV; v++) {
+                            dephase_relax(m[v], ph[v].angle(), T1, T2, te);
+                            const RfPulse p = view_rf(a, xv, v)[rf_i];
+                            if (sizeof(R) == 8) rot_apply<R>(p.kind_d, (R)p.sd, (R)p.cd, (R)p.spd, (R)p.cpd, m[v]);
+                            else                rot_apply<R>(p.kind, (R)p.s, (R)p.c, (R)p.sp, (R)p.cp, m[v]);
+                            ph[v].reset();
+                        }
+                        old_tp = tp;
+                        ++rf_i;
+                        next_rf = (rf_i < a.n_rf) ? (uint32_t)a.rf[rf_i].t : kNever;
+                    }
+                    if (tp == next_te) {                           // :202-215
+                        const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);
+                        const uint64_t o = (uint64_t)li * a.n_te + te_i;
+                        #pragma unroll
+                        for (int v = 0; v < NV; v++) {
+                            dephase_relax(m[v], ph[v].angle(), T1, T2, te);
+                            R *M1 = (R *)view_M1(a, xv, v);
+                            M1[3 * o + 0] = m[v][0]; M1[3 * o + 1] = m[v][1]; M1[3 * o + 2] = m[v][2];
+                            view_T(a, xv, v)[o] = (uint8_t)ts_old;
+                            ph[v].reset();
+                        }
+                        old_tp = tp;
+                        ++te_i;
+                        next_te = (te_i < a.n_te) ? (uint32_t)a.TE_t[te_i] : kNever;
+                    }
+                    next_ev = next_event(next_rf, next_te, next_dp, next_gr);
+                }
+                if (a.record_traj) {                               // :218-221
+                    #pragma unroll
+                    for (int v = 0; v < NV; v++) {
+                        float *dst = xyz1_of(v) + 3 * slot;
+                        dst[0] = P::metres(a, xn[0], 0); dst[1] = P::metres(a, xn[1], 1);
+                        dst[2] = P::metres(a, xn[2], 2);
+                    }
+                    ++slot;
+                }
+                xo[0] = xn[0]; xo[1] = xn[1]; xo[2] = xn[2];
+                ++tp;
+            }
+            return acc;
+        };
+
+        while (tp < a.n_timepoints) {                              // :128
+            const bool moved = stp.moves();
+            const Vox<IDX32> va = P::template advance<Rec, CROSS, IDX32>(a, moved, q0.x, q0.y, q0.z, stp, xo, xa);
+            const bool new_a = va.ok && va.rec != rec_old;         // :150
+            typename Rec::T ra = rec_load<Rec, IDX32>(a.rec, new_a ? va.rec : (RecIdx<IDX32>)0);
+            const Vox<IDX32> vb = P::template advance<Rec, CROSS, IDX32>(a, moved, q1.x, q1.y, q1.z, stp, xa, xb);
+            const bool new_b = va.ok && vb.ok && vb.rec != va.rec;
+            typename Rec::T rb = rec_load<Rec, IDX32>(a.rec, new_b ? vb.rec : (RecIdx<IDX32>)0);
+            const Draw q2 = rng.draw(att + 2), q3 = rng.draw(att + 3);   // while the gathers fly
+            after_draws(ra, rb, q2, q3);
+            bool kept = true;
+            const bool acc_a = attempt(va, ra, new_a, xa, q0.u, kept);
+            // After a step A the voxel of the spin is va's, so B's gather is
+            // needed exactly when vb differs from it (new_b).
+            const bool both = acc_a && kept && tp < a.n_timepoints;
+            if (both) attempt(vb, rb, new_b, xb, q1.u, kept);
+            att += both ? 2 : 1;
+            const Draw n0 = both ? q2 : q1, n1 = both ? q3 : q2;
+            q0 = n0; q1 = n1;
+        }
+        if (tp & kParked) { st = ST_LOST; tp &= ~kParked; goto done; }
+        const R te = (R)((tp - old_tp) * (uint32_t)a.timestep_us * 1e-6);   // :226-231
+        #pragma unroll
+        for (int v = 0; v < NV; v++) dephase_relax(m[v], ph[v].angle(), T1, T2, te);
+    }
+    d = a.n_dummy + 1;                                             // every TR done
+    tp = 0;
+done:
+    if (!a.record_traj) {
+        #pragma unroll
+        for (int v = 0; v < NV; v++)
+            for (int i = 0; i < 3; i++) xyz1_of(v)[i] = P::metres(a, xo[i], i);
+    }
+    n_acc += (uint64_t)d * a.n_timepoints + tp;
+    return st;
+}
+
 
 // One workgroup walks the spins at positions [t0, t0 + blockDim.x) of the
 // launch's visiting order (fewer at the end).
@@ -1173,16 +1420,22 @@ __device__ __forceinline__ void walk_block_views(const WalkArgs &a, const WalkVi
 #ifndef SW_WALK_WAVES_PER_EU
 #define SW_WALK_WAVES_PER_EU 6
 #endif
+#ifndef SW_PAIR_WAVES_PER_EU
+#define SW_PAIR_WAVES_PER_EU 4
+#endif
 
-template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SW_WALK_WAVES_PER_EU)))
+// The two-attempt loop holds a second position, record and two draw bundles
+// more: it is compiled for 4 waves per SIMD (up to 128 registers), which it
+// keeps busy with two gathers in flight per wave.
+template <class Rng, class Rec, class Phase, bool CROSS, bool IDX32, bool PAIRS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PAIRS ? SW_PAIR_WAVES_PER_EU : SW_WALK_WAVES_PER_EU)))
 void walk_kernel(const WalkArgs a) {
     // Blocks are dealt round-robin over the 8 XCDs; remap so that each XCD gets a
     // contiguous range of the visiting order (spatially sorted spins share L2).
     // The grid is a multiple of 8 blocks, so the remap is a permutation.
     const uint32_t per = gridDim.x / 8, b = blockIdx.x;
     const uint32_t lb = (b % 8) * per + b / 8;
-    walk_block<Rng, Rec, Phase, CROSS, IDX32>(a, lb * blockDim.x);
+    walk_block<Rng, Rec, Phase, CROSS, IDX32, PAIRS>(a, lb * blockDim.x);
 }
 
 // The batched walk: block b takes the chunk at slot (b mod 8) * (grid / 8) + b / 8
@@ -1233,22 +1486,31 @@ WalkMode mode_of(const WalkArgs &a) {
     WalkMode m;
     m.idx32 = bytes < (1ull << 32) && (n_tab + tissue_lds_words(a)) * 4 <= 65536;
     m.lds = m.idx32 ? (size_t)(n_tab + tissue_lds_words(a)) * 4 : 0;
+    m.bytes = bytes;
     return m;
 }
 
-template <class Rng, class F, class P>
-hipError_t launch_fp(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
+template <class Rng, class F, class P, bool PAIRS>
+hipError_t launch_fp_loop(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
     const WalkMode m = mode_of<F>(a);
     const bool idx32 = m.idx32;
     const size_t lds = m.lds;
     if (a.cross_fov) {
-        if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, true>), grid, block, lds, s, a);
-        else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, false>), grid, block, 0, s, a);
+        if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, true, PAIRS>), grid, block, lds, s, a);
+        else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, false, PAIRS>), grid, block, 0, s, a);
     } else {
-        if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, false, true>), grid, block, lds, s, a);
-        else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, false, false>), grid, block, 0, s, a);
+        if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, false, true, PAIRS>), grid, block, lds, s, a);
+        else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, false, false, PAIRS>), grid, block, 0, s, a);
     }
     return hipGetLastError();
+}
+
+template <class Rng, class F, class P>
+hipError_t launch_fp(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
+    if constexpr (!Rng::kReplay) {
+        if (a.pairs) return launch_fp_loop<Rng, F, P, true>(a, s, grid, block);
+    }
+    return launch_fp_loop<Rng, F, P, false>(a, s, grid, block);
 }
 
 template <class Rng, class F, class P, int NV>

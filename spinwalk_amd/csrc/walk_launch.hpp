@@ -14,7 +14,7 @@ hipError_t launch_walk(const WalkArgs &a, int rng, int field, int prec, hipStrea
 
 // The index mode the walk takes for a phantom and field format: 32-bit record
 // offsets with per-axis LDS tables of `lds` bytes, or 64-bit arithmetic.
-struct WalkMode { bool idx32; size_t lds; };
+struct WalkMode { bool idx32; size_t lds; uint64_t bytes; };   // bytes: the gathered records
 WalkMode walk_mode(const WalkArgs &a, int field, int rng);
 
 // The batched walk over a chunk table of n_slots entries (a multiple of 8),

@@ -28,6 +28,7 @@ SW_RNG_PHILOX, SW_RNG_REPLAY = 0, 1
 SW_PREC_F32, SW_PREC_F32_KAHAN, SW_PREC_F64 = 0, 1, 2
 SW_FIELD_NONE, SW_FIELD_F32, SW_FIELD_F16, SW_FIELD_BF16, SW_FIELD_Q16 = 0, 1, 2, 3, 4
 SW_DTYPE_F32, SW_DTYPE_F64 = 0, 1
+SW_STEP_LOOP_AUTO, SW_STEP_LOOP_ONE, SW_STEP_LOOP_TWO = 0, 1, 2
 
 _P = C.c_void_p
 
@@ -68,6 +69,7 @@ class SwWalkOpts(C.Structure):
         ("spatial_order", C.c_int32),
         ("replay_normals", _P), ("replay_normal_stride", C.c_uint64),
         ("replay_uniforms", _P), ("replay_uniform_stride", C.c_uint64),
+        ("step_loop", C.c_int32),
     ]
 
 
@@ -509,12 +511,13 @@ class Engine:
     # ---- walks
     def walk_host(self, ph, seq, params, M0, XYZ0, M1, XYZ1, T, *, spin_offset=0, xyz_scale=1.0,
                   gradient_scale=1.0, rng=SW_RNG_PHILOX, precision=SW_PREC_F32, field_format=SW_FIELD_F32,
-                  spatial_order=0, replay=None):
+                  spatial_order=0, replay=None, step_loop=SW_STEP_LOOP_AUTO):
         """Host arrays in, host arrays out (in place), reference semantics."""
         o = SwWalkOpts()
         o.spin_offset, o.n_local = int(spin_offset), int(M0.shape[0])
         o.xyz_scale, o.gradient_scale = float(xyz_scale), float(gradient_scale)
         o.rng, o.precision, o.field_format, o.spatial_order = rng, precision, field_format, spatial_order
+        o.step_loop = int(step_loop)
         rn = ru = None
         if replay is not None:
             rn = np.ascontiguousarray(replay[0], np.float32)
@@ -533,15 +536,18 @@ class Engine:
 
     def walk_device(self, ph, seq, params, M0, XYZ0, M1, XYZ1, T, *, spin_offset=0, n_local=None, xyz_scale=1.0,
                     gradient_scale=1.0, rng=SW_RNG_PHILOX, precision=SW_PREC_F32, field_format=SW_FIELD_F32,
-                    spatial_order=0, replay=None, stream=None, stats=True, noalloc=False):
+                    spatial_order=0, replay=None, stream=None, stats=True, noalloc=False,
+                    step_loop=SW_STEP_LOOP_AUTO):
         """torch device tensors; launches on `stream` (torch stream or None).
         noalloc: walk through sw_walk_device_noalloc (fixed device scratch, no
-        allocation) instead of sw_walk_device."""
+        allocation) instead of sw_walk_device.  step_loop: SW_STEP_LOOP_* (the
+        production walk's one- or two-attempt step loop; same results)."""
         o = SwWalkOpts()
         o.spin_offset = int(spin_offset)
         o.n_local = int(M0.shape[0] if n_local is None else n_local)
         o.xyz_scale, o.gradient_scale = float(xyz_scale), float(gradient_scale)
         o.rng, o.precision, o.field_format, o.spatial_order = rng, precision, field_format, spatial_order
+        o.step_loop = int(step_loop)
         self._check_spin_buffers(seq, params, o.n_local, precision, M0, XYZ0, M1, XYZ1, T, replay, rng,
                                  exact=n_local is None)
         for t in (M0, XYZ0, M1, XYZ1, T) + (tuple(replay) if replay is not None else ()):

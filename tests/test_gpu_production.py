@@ -31,7 +31,7 @@ GAMMA = 267515315.0
 
 
 def run(engine, case, scale_index=0, precision=E.SW_PREC_F32, field_format=E.SW_FIELD_F32, spatial_order=1,
-        spin_offset=0, n=None, formats=(E.SW_FIELD_F32,)):
+        spin_offset=0, n=None, formats=(E.SW_FIELD_F32,), step_loop=E.SW_STEP_LOOP_AUTO):
     p, xs, gs = case.scale_params(case.scales[scale_index])
     n = case.n_spins - spin_offset if n is None else n
     sl = slice(spin_offset, spin_offset + n)
@@ -43,7 +43,7 @@ def run(engine, case, scale_index=0, precision=E.SW_PREC_F32, field_format=E.SW_
     XYZ1 = np.zeros((n, case.traj_len, 3), np.float32)
     st = engine.walk_host(ph, seq, p, np.ascontiguousarray(case.M0[sl]), np.ascontiguousarray(case.XYZ0[sl]), M1,
                           XYZ1, T, spin_offset=spin_offset, xyz_scale=xs, gradient_scale=gs, precision=precision,
-                          field_format=field_format, spatial_order=spatial_order)
+                          field_format=field_format, spatial_order=spatial_order, step_loop=step_loop)
     return dict(M1=M1, T=T, XYZ1=XYZ1, stats=st)
 
 
