@@ -41,6 +41,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 PRECISIONS = {"f32": 0, "kahan": 1, "f64": 2}
 FIELDS = {"f32": 1, "f16": 2, "bf16": 3, "q16": 4}
+STEP_LOOPS = {"auto": 0, "one": 1, "two": 2}
 
 
 def log(*a):
@@ -178,7 +179,8 @@ def main_tiles(args, torch, E, W, eng, stream, dev):
         for t in seq_tiles:
             eng.walk_device(t["phantom"], t["sequence"], t["params"], t["M0"], t["XYZ0"], t["M1"], t["XYZ1"], t["T"],
                             xyz_scale=t["xyz_scale"], gradient_scale=t["gradient_scale"], precision=0,
-                            field_format=E.SW_FIELD_F32, spatial_order=0, stream=stream, stats=False)
+                            field_format=E.SW_FIELD_F32, spatial_order=0, stream=stream, stats=False,
+                            step_loop=STEP_LOOPS[args.step_loop])
             eng.echo_reduce(t["M1"], t["T"], t["params"].n_substrate, t["echo_sums"], stream=stream)
 
     def timed(fn):
@@ -380,6 +382,8 @@ def main():
                     help="single-walk workloads: the field format (default f32); tile workloads: the mixed "
                          "batch's (default q16)")
     ap.add_argument("--order", type=int, default=1, help="spatial visiting order (1) or spin order (0)")
+    ap.add_argument("--step-loop", default="auto", choices=sorted(STEP_LOOPS),
+                    help="step loop of the single walks: one or two attempts per trip, or auto (by record size)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="wall-time budget of the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--host-io", action="store_true",
@@ -440,7 +444,8 @@ def main():
 
     def one_step():
         st = eng.walk_device(ph, seq, p, M0, XYZ0, M1, XYZ1, T, rng=E.SW_RNG_PHILOX, precision=prec,
-                             field_format=fmt, spatial_order=args.order, stream=stream, stats=True)
+                             field_format=fmt, spatial_order=args.order, stream=stream, stats=True,
+                             step_loop=STEP_LOOPS[args.step_loop])
         eng.echo_reduce(M1, T, p.n_substrate, sums, stream=stream)
         return st
 
@@ -521,7 +526,7 @@ def main():
                    if args.workload == "c2" else str(wl["phantom"]["dims"]), "spins": n,
                    "steps_per_spin": case.steps_per_spin(), "sequence": "SE 90/180 TE 40 ms dt 20 us"
                    if args.workload == "c2" else "GRE", "precision": args.precision, "field": args.field,
-                   "spatial_order": args.order, "rng": "philox4x32-10"},
+                   "spatial_order": args.order, "rng": "philox4x32-10", "step_loop": args.step_loop},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      # the measured line bytes of one launch (rocprof FETCH_SIZE + WRITE_SIZE, calibrated)

@@ -75,7 +75,7 @@ struct sw_phantom {
     bool has_field = false;
     // bricked voxel records gathered by the walk (walk_args.hpp)
     void *rec[5] = {};               // by SW_FIELD_*: none (label only), f32, f16, bf16, q16
-    void *rec_soa = nullptr;         // float32 structure-of-arrays bricks (production walks)
+    void *rec_packed = nullptr;      // float32 packed 5-byte records (production walks)
 };
 
 struct sw_sequence {
@@ -379,7 +379,7 @@ static size_t phantom_bytes(uint64_t n, bool has_field, uint32_t formats) {
     if (!has_field) return need + n * 4;
     need += n * 4;
     for (int f = SW_FIELD_F32; f <= SW_FIELD_Q16; f++)
-        if (formats & (1u << f)) need += n * (f == SW_FIELD_F32 ? 8 + 6 : 4);   // f32: AoS and SoA bricks
+        if (formats & (1u << f)) need += n * (f == SW_FIELD_F32 ? 8 + 6 : 4);   // f32: 8-byte and packed records
     return need;
 }
 
@@ -420,13 +420,13 @@ static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
         const int kind = f == SW_FIELD_F32 ? 0 : f == SW_FIELD_F16 ? 1 : f == SW_FIELD_BF16 ? 2 : f == SW_FIELD_Q16 ? 4 : 3;
         SW_HIP(e, sw::launch_records(ph->mask, ph->field, nx, ny, nz, kind,
                                      1.0f / (f == SW_FIELD_Q16 ? ph->q_scale : ph->half_scale), ph->rec[f], s));
-        if (f == SW_FIELD_F32) {         // the production walk's 5-byte structure-of-arrays copy
-            const size_t bytes = (size_t)(sw::soa_brick_count(nx, sw::kSoaF32.bx) *
-                                          sw::soa_brick_count(ny, sw::kSoaF32.by) *
-                                          sw::soa_brick_count(nz, sw::kSoaF32.bz)) * 128;
-            SW_HIP(e, hipMalloc(&ph->rec_soa, bytes));
-            SW_HIP(e, hipMemsetAsync(ph->rec_soa, 0, bytes, s));
-            SW_HIP(e, sw::launch_soa_records(ph->mask, ph->field, nx, ny, nz, ph->rec_soa, s));
+        if (f == SW_FIELD_F32) {         // the production walk's packed 5-byte copy
+            const size_t bytes = (size_t)(sw::pack_brick_count(nx, sw::kPackF32.bx) *
+                                          sw::pack_brick_count(ny, sw::kPackF32.by) *
+                                          sw::pack_brick_count(nz, sw::kPackF32.bz)) * 128;
+            SW_HIP(e, hipMalloc(&ph->rec_packed, bytes));
+            SW_HIP(e, hipMemsetAsync(ph->rec_packed, 0, bytes, s));
+            SW_HIP(e, sw::launch_packed_records(ph->mask, ph->field, nx, ny, nz, ph->rec_packed, s));
         }
     }
     SW_HIP(e, hipStreamSynchronize(s));
@@ -478,7 +478,7 @@ SW_API void sw_phantom_destroy(sw_phantom *ph) {
     dev_free(ph->mask);
     dev_free(ph->field);
     for (void *r : ph->rec) dev_free(r);
-    dev_free(ph->rec_soa);
+    dev_free(ph->rec_packed);
     delete ph;
 }
 
@@ -790,11 +790,11 @@ int fill_walk_args(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, con
     std::memset(&a, 0, sizeof a);
     a.rec = ph->rec[field];
     if (field == SW_FIELD_F32 && o->rng == SW_RNG_PHILOX) {
-        // production walks gather the float32 field from the structure-of-arrays bricks
-        if (!ph->rec_soa) return fail(e, SW_E_INVALID, "the phantom has no float32 structure-of-arrays records");
-        a.rec = ph->rec_soa;
-        a.nb_y = (uint32_t)sw::soa_brick_count(ph->dims[1], sw::kSoaF32.by);
-        a.nb_z = (uint32_t)sw::soa_brick_count(ph->dims[2], sw::kSoaF32.bz);
+        // production walks gather the float32 field from the packed 5-byte records
+        if (!ph->rec_packed) return fail(e, SW_E_INVALID, "the phantom has no packed float32 records");
+        a.rec = ph->rec_packed;
+        a.nb_y = (uint32_t)sw::pack_brick_count(ph->dims[1], sw::kPackF32.by);
+        a.nb_z = (uint32_t)sw::pack_brick_count(ph->dims[2], sw::kPackF32.bz);
     } else {
         const sw::BrickShape sh = field == SW_FIELD_F32 ? sw::kBrick8 : sw::kBrick4;
         a.nb_y = (uint32_t)sw::brick_count(ph->dims[1], sh.lby);

@@ -69,30 +69,31 @@ __global__ void records_kernel(const uint8_t *mask, const float *field, uint64_t
     }
 }
 
-// Row-major mask/field -> structure-of-arrays float32 bricks (kSoaF32).
-__global__ void soa_records_kernel(const uint8_t *mask, const float *field, uint64_t n, uint32_t ny, uint32_t nz,
-                                   uint64_t nb_y, uint64_t nb_z, uint8_t *out) {
-    constexpr SoaShape S = kSoaF32;
+// Row-major mask/field -> packed float32 bricks (kPackF32).
+__global__ void packed_records_kernel(const uint8_t *mask, const float *field, uint64_t n, uint32_t ny, uint32_t nz,
+                                      uint64_t nb_y, uint64_t nb_z, uint8_t *out) {
+    constexpr PackShape S = kPackF32;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t iz = v % nz, iy = (v / nz) % ny, ix = v / ((uint64_t)nz * ny);
         const uint64_t b = ((ix / S.bx) * nb_y + iy / S.by) * nb_z + iz / S.bz;
         const uint32_t j = (uint32_t)(((ix % S.bx) * S.by + iy % S.by) * S.bz + iz % S.bz);
-        uint8_t *brick = out + b * 128;
-        brick[j] = mask[v];
-        *(uint32_t *)(brick + S.nvox + S.field_bytes * j) = __float_as_uint(field ? field[v] : 0.f);
+        uint8_t *r = out + b * 128 + S.rec_bytes * j;
+        const uint32_t f = __float_as_uint(field ? field[v] : 0.f);
+        r[0] = mask[v];
+        r[1] = (uint8_t)f; r[2] = (uint8_t)(f >> 8); r[3] = (uint8_t)(f >> 16); r[4] = (uint8_t)(f >> 24);
     }
 }
 
 }  // namespace
 
-hipError_t launch_soa_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz,
+hipError_t launch_packed_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz,
                               void *out, hipStream_t s) {
     const uint64_t n = nx * ny * nz;
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 16384) blocks = 16384;
-    const uint64_t nby = soa_brick_count(ny, kSoaF32.by), nbz = soa_brick_count(nz, kSoaF32.bz);
-    hipLaunchKernelGGL(soa_records_kernel, dim3((unsigned)blocks), dim3(256), 0, s, mask, field, n, (uint32_t)ny,
+    const uint64_t nby = pack_brick_count(ny, kPackF32.by), nbz = pack_brick_count(nz, kPackF32.bz);
+    hipLaunchKernelGGL(packed_records_kernel, dim3((unsigned)blocks), dim3(256), 0, s, mask, field, n, (uint32_t)ny,
                        (uint32_t)nz, nby, nbz, (uint8_t *)out);
     return hipGetLastError();
 }

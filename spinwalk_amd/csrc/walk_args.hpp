@@ -47,15 +47,16 @@ struct BrickShape { int lbx, lby, lbz; };
 constexpr BrickShape kBrick8 = {2, 1, 1};
 constexpr BrickShape kBrick4 = {2, 2, 1};
 
-// Structure-of-arrays records of the production walk's float32 field: a
-// 128-byte brick of 2 x 3 x 4 voxels holds their 24 labels (bytes 0-23), then
-// their 24 fields (bytes 24-119, 8 bytes of padding).  5 bytes a voxel instead
-// of 8, so a line covers 24 voxels instead of 16.  Voxel (x, y, z) of a brick
-// is slot j = (x * 3 + y) * 4 + z; its label sits at byte j of the brick and
-// its field at byte 24 + 4 j.  Bricks are row-major, as for BrickShape.
-struct SoaShape { int bx, by, bz, nvox, field_bytes; };
-constexpr SoaShape kSoaF32 = {2, 3, 4, 24, 4};
-SW_HD_INLINE uint64_t soa_brick_count(uint64_t n, int b) { return (n + b - 1) / b; }
+// Packed records of the production walk's float32 field: a 128-byte line holds
+// a brick of 2 x 3 x 4 voxels as 24 records of 5 bytes, {label, float32 field}
+// at bytes 5 j .. 5 j + 4 of the line for voxel (x, y, z) of the brick, slot
+// j = (x * 3 + y) * 4 + z (bytes 120-127 are padding).  5 bytes a voxel instead
+// of 8, so a line covers 24 voxels instead of 16; and one 8-byte load at byte
+// 5 j (unaligned, within the line: it ends by byte 122) reads label and field
+// together.  Bricks are row-major, as for BrickShape.
+struct PackShape { int bx, by, bz, nvox, rec_bytes; };
+constexpr PackShape kPackF32 = {2, 3, 4, 24, 5};
+SW_HD_INLINE uint64_t pack_brick_count(uint64_t n, int b) { return (n + b - 1) / b; }
 
 SW_HD_INLINE uint64_t brick_count(uint64_t n, int lb) { return (n + (1ull << lb) - 1) >> lb; }
 
@@ -82,7 +83,7 @@ struct WalkView {
 
 struct WalkArgs {
     // ---- phantom -----------------------------------------------------------
-    const void     *rec;          // bricked voxel records (Rec8, Rec4 or the SoA float32 bricks)
+    const void     *rec;          // bricked voxel records (Rec8, Rec4 or the packed float32 bricks)
     uint32_t        nb_y, nb_z;   // bricks along y and z
     float           t2d;          // float T -> deg/step factor (monte_carlo.cu:241)
     float           h2d;          // half -> deg/step factor = half_scale * t2d

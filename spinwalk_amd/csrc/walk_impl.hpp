@@ -162,7 +162,7 @@ __device__ __forceinline__ T load_rec(const void *base, uint64_t i) {
 
 struct RecF32 {                          // {float field_T, uint32 label}
     static constexpr BrickShape shape = kBrick8;
-    static constexpr bool kSoA = false;
+    static constexpr bool kPacked = false;
     using T = uint2;
     __device__ static uint32_t label(T r) { return r.y; }
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
@@ -172,14 +172,14 @@ struct RecF32 {                          // {float field_T, uint32 label}
 };
 struct RecNone {                         // label only: {0, uint8 label, pad}
     static constexpr BrickShape shape = kBrick4;
-    static constexpr bool kSoA = false;
+    static constexpr bool kPacked = false;
     using T = uint32_t;
     __device__ static uint32_t label(T r) { return r >> 16; }
     template <class R> __device__ static R field(const WalkArgs &, T) { return (R)0; }
 };
 struct RecF16 {                          // {fp16 field_T/half_scale, uint8 label, pad}
     static constexpr BrickShape shape = kBrick4;
-    static constexpr bool kSoA = false;
+    static constexpr bool kPacked = false;
     using T = uint32_t;
     __device__ static uint32_t label(T r) { return r >> 16; }
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
@@ -188,7 +188,7 @@ struct RecF16 {                          // {fp16 field_T/half_scale, uint8 labe
 };
 struct RecQ16 {                          // {int16 rint(field_T/q_scale), uint8 label, pad}
     static constexpr BrickShape shape = kBrick4;
-    static constexpr bool kSoA = false;
+    static constexpr bool kPacked = false;
     using T = uint32_t;
     __device__ static uint32_t label(T r) { return r >> 16; }
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
@@ -197,7 +197,7 @@ struct RecQ16 {                          // {int16 rint(field_T/q_scale), uint8 
 };
 struct RecBF16 {                         // {bf16 field_T/half_scale, uint8 label, pad}
     static constexpr BrickShape shape = kBrick4;
-    static constexpr bool kSoA = false;
+    static constexpr bool kPacked = false;
     using T = uint32_t;
     __device__ static uint32_t label(T r) { return r >> 16; }
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
@@ -205,28 +205,30 @@ struct RecBF16 {                         // {bf16 field_T/half_scale, uint8 labe
     }
 };
 
-// The production walk's float32 records: structure-of-arrays bricks of 2x3x4
-// voxels (kSoaF32).  A record index is the byte offset of the voxel's label;
-// its field sits 24 + 3 j bytes further, j = the slot in the brick = offset
-// mod 128.  Two loads from one line.
-struct RecF32S {
-    static constexpr bool kSoA = true;
-    static constexpr SoaShape soa = kSoaF32;
-    using T = uint2;                     // {field bits, label}, as RecF32
-    __device__ static uint32_t label(T r) { return r.y; }
-    template <class R> __device__ static R field(const WalkArgs &a, T r) { return RecF32::field<R>(a, r); }
+// The production walk's float32 records: packed 5-byte records in bricks of
+// 2x3x4 voxels, one 128-byte line each (kPackF32).  A record index is the byte
+// offset of the record; one 8-byte load there (unaligned, inside the line)
+// brings the label (byte 0) and the field (bytes 1-4).
+typedef uint64_t __attribute__((aligned(1))) u64_unaligned;
+struct RecF32P {
+    static constexpr bool kPacked = true;
+    static constexpr PackShape pack = kPackF32;
+    using T = uint2;                     // the 8 bytes loaded: x = bytes 0-3, y = bytes 4-7
+    __device__ static uint32_t label(T r) { return r.x & 0xFFu; }
+    template <class R> __device__ static R field(const WalkArgs &a, T r) {
+        uint2 f;
+        f.x = __builtin_amdgcn_alignbit(r.y, r.x, 8);                      // bytes 1-4
+        return RecF32::field<R>(a, f);
+    }
     template <bool IDX32, class I> __device__ static T load(const void *base, I off) {
-        const I foff = off + (I)(soa.field_bytes - 1) * (off & (I)127);   // field - nvox, as an offset
-        T r;
-        r.y = gload<uint8_t>(base, off);
-        r.x = gload<uint32_t>((const char *)base + soa.nvox, foff);
-        return r;
+        const uint64_t v = gload<u64_unaligned>(base, off);
+        return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
     }
 };
 
 template <class Rec, bool IDX32, class I>
 __device__ __forceinline__ typename Rec::T rec_load(const void *base, I off) {
-    if constexpr (Rec::kSoA) return Rec::template load<IDX32>(base, off);
+    if constexpr (Rec::kPacked) return Rec::template load<IDX32>(base, off);
     else return load_rec<typename Rec::T, IDX32>(base, off);
 }
 
@@ -328,24 +330,25 @@ template <bool IDX32> struct Vox {
 // axis_tab[0, nx) = X, [nx, nx+ny) = Y, [nx+ny, nx+ny+nz) = Z.
 extern __shared__ uint32_t axis_tab[];
 
-// For the structure-of-arrays bricks the index is the label's byte offset,
-// separable in the same way: X = (ix / bx) * nb_y * nb_z * 128 + (ix mod bx) * by * bz,
-// Y = (iy / by) * nb_z * 128 + (iy mod by) * bz, Z = (iz / bz) * 128 + iz mod bz.
+// For the packed bricks the index is the record's byte offset, separable in the
+// same way (w = 5 bytes a record):
+// X = (ix / bx) * nb_y * nb_z * 128 + (ix mod bx) * by * bz * w,
+// Y = (iy / by) * nb_z * 128 + (iy mod by) * bz * w, Z = (iz / bz) * 128 + (iz mod bz) * w.
 template <class Rec, class I>
-__device__ __forceinline__ I soa_part(const WalkArgs &a, int axis, I i) {
-    constexpr SoaShape S = Rec::soa;
-    if (axis == 0) return (i / S.bx) * (I)a.nb_y * (I)a.nb_z * 128 + (i % S.bx) * (S.by * S.bz);
-    if (axis == 1) return (i / S.by) * (I)a.nb_z * 128 + (i % S.by) * S.bz;
-    return (i / S.bz) * 128 + i % S.bz;
+__device__ __forceinline__ I pack_part(const WalkArgs &a, int axis, I i) {
+    constexpr PackShape S = Rec::pack;
+    if (axis == 0) return (i / S.bx) * (I)a.nb_y * (I)a.nb_z * 128 + (i % S.bx) * (S.by * S.bz * S.rec_bytes);
+    if (axis == 1) return (i / S.by) * (I)a.nb_z * 128 + (i % S.by) * (S.bz * S.rec_bytes);
+    return (i / S.bz) * 128 + (i % S.bz) * S.rec_bytes;
 }
 
 template <class Rec>
 __device__ __forceinline__ uint32_t axis_entry(const WalkArgs &a, uint32_t i) {
     const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny;
-    if constexpr (Rec::kSoA) {
-        if (i < nx) return soa_part<Rec, uint32_t>(a, 0, i);
-        if (i < nx + ny) return soa_part<Rec, uint32_t>(a, 1, i - nx);
-        return soa_part<Rec, uint32_t>(a, 2, i - nx - ny);
+    if constexpr (Rec::kPacked) {
+        if (i < nx) return pack_part<Rec, uint32_t>(a, 0, i);
+        if (i < nx + ny) return pack_part<Rec, uint32_t>(a, 1, i - nx);
+        return pack_part<Rec, uint32_t>(a, 2, i - nx - ny);
     } else {
         constexpr int LX = Rec::shape.lbx, LY = Rec::shape.lby, LZ = Rec::shape.lbz, L = LX + LY + LZ;
         if (i < nx)
@@ -365,8 +368,8 @@ __device__ __forceinline__ uint64_t record_of(const WalkArgs &a, uint64_t ix, ui
         const uint32_t nx = (uint32_t)a.nx, nxy = nx + (uint32_t)a.ny;
         return axis_tab[(uint32_t)ix] + axis_tab[nx + (uint32_t)iy] + axis_tab[nxy + (uint32_t)iz];
     }
-    if constexpr (Rec::kSoA) {
-        return soa_part<Rec, uint64_t>(a, 0, ix) + soa_part<Rec, uint64_t>(a, 1, iy) + soa_part<Rec, uint64_t>(a, 2, iz);
+    if constexpr (Rec::kPacked) {
+        return pack_part<Rec, uint64_t>(a, 0, ix) + pack_part<Rec, uint64_t>(a, 1, iy) + pack_part<Rec, uint64_t>(a, 2, iz);
     } else {
         constexpr int LX = Rec::shape.lbx, LY = Rec::shape.lby, LZ = Rec::shape.lbz;
         const uint64_t inner = ((ix & ((1u << LX) - 1)) << (LY + LZ)) | ((iy & ((1u << LY) - 1)) << LZ) |
@@ -1476,8 +1479,8 @@ void walk_batch_kernel(const void *__restrict__ items, const BatchChunk *__restr
 template <class F>
 WalkMode mode_of(const WalkArgs &a) {
     uint64_t bytes;
-    if constexpr (F::kSoA) {
-        bytes = soa_brick_count(a.nx, F::soa.bx) * a.nb_y * a.nb_z * 128;
+    if constexpr (F::kPacked) {
+        bytes = pack_brick_count(a.nx, F::pack.bx) * a.nb_y * a.nb_z * 128;
     } else {
         constexpr int lb = F::shape.lbx + F::shape.lby + F::shape.lbz;
         bytes = (brick_count(a.nx, F::shape.lbx) * a.nb_y * a.nb_z << lb) * sizeof(typename F::T);
@@ -1540,8 +1543,8 @@ hipError_t launch_batch_fp(const void *items, const BatchChunk *chunks, uint32_t
 }
 
 // The float32 records a walk gathers: the reference layout for replayed draws
-// (held bit for bit to the oracle), the structure-of-arrays bricks in production.
-template <class Rng> using RecF32Of = typename std::conditional<Rng::kReplay, RecF32, RecF32S>::type;
+// (held bit for bit to the oracle), the packed 5-byte records in production.
+template <class Rng> using RecF32Of = typename std::conditional<Rng::kReplay, RecF32, RecF32P>::type;
 
 template <class Rng>
 hipError_t launch_batch_rng(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,

@@ -47,9 +47,9 @@ hipError_t launch_phantom_stats(const uint8_t *mask, const float *field, uint64_
 // padding.
 hipError_t launch_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz, int kind,
                           float inv_scale, void *out, hipStream_t s);
-// Row-major mask/field -> the structure-of-arrays float32 bricks (kSoaF32) of
+// Row-major mask/field -> the packed float32 bricks (kPackF32) of
 // the production walk; `out` must be zero-filled for padding.
-hipError_t launch_soa_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz,
+hipError_t launch_packed_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz,
                               void *out, hipStream_t s);
 // FP64 echo reduction into out[(te*n_sub+label)*3 + {0,1,2}]; partial is scratch
 // of reduce_partial_len(n_spins, n_te, n_sub) doubles.
