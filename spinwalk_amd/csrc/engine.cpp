@@ -411,12 +411,17 @@ static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
     for (int f = 0; f <= SW_FIELD_Q16; f++) {
         const bool want = (f == SW_FIELD_NONE) ? !ph->has_field : (ph->has_field && (formats & (1u << f)));
         if (!want) continue;
-        const sw::BrickShape sh = f == SW_FIELD_F32 ? sw::kBrick8 : sw::kBrick4;
-        const size_t recsz = f == SW_FIELD_F32 ? 8 : 4;
-        const size_t nrec = (size_t)(sw::brick_count(nx, sh.lbx) * sw::brick_count(ny, sh.lby) *
-                                     sw::brick_count(nz, sh.lbz)) << (sh.lbx + sh.lby + sh.lbz);
-        SW_HIP(e, hipMalloc(&ph->rec[f], nrec * recsz));
-        SW_HIP(e, hipMemsetAsync(ph->rec[f], 0, nrec * recsz, s));
+        size_t bytes;
+        if (f == SW_FIELD_F16 || f == SW_FIELD_BF16 || f == SW_FIELD_Q16) {   // packed 3-byte records
+            bytes = (size_t)(sw::pack_brick_count(nx, sw::kPack16.bx) * sw::pack_brick_count(ny, sw::kPack16.by) *
+                             sw::pack_brick_count(nz, sw::kPack16.bz)) * 128;
+        } else {
+            const sw::BrickShape sh = f == SW_FIELD_F32 ? sw::kBrick8 : sw::kBrick4;
+            bytes = ((size_t)(sw::brick_count(nx, sh.lbx) * sw::brick_count(ny, sh.lby) * sw::brick_count(nz, sh.lbz))
+                     << (sh.lbx + sh.lby + sh.lbz)) * (f == SW_FIELD_F32 ? 8 : 4);
+        }
+        SW_HIP(e, hipMalloc(&ph->rec[f], bytes));
+        SW_HIP(e, hipMemsetAsync(ph->rec[f], 0, bytes, s));
         const int kind = f == SW_FIELD_F32 ? 0 : f == SW_FIELD_F16 ? 1 : f == SW_FIELD_BF16 ? 2 : f == SW_FIELD_Q16 ? 4 : 3;
         SW_HIP(e, sw::launch_records(ph->mask, ph->field, nx, ny, nz, kind,
                                      1.0f / (f == SW_FIELD_Q16 ? ph->q_scale : ph->half_scale), ph->rec[f], s));
@@ -795,6 +800,9 @@ int fill_walk_args(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, con
         a.rec = ph->rec_packed;
         a.nb_y = (uint32_t)sw::pack_brick_count(ph->dims[1], sw::kPackF32.by);
         a.nb_z = (uint32_t)sw::pack_brick_count(ph->dims[2], sw::kPackF32.bz);
+    } else if (field == SW_FIELD_F16 || field == SW_FIELD_BF16 || field == SW_FIELD_Q16) {
+        a.nb_y = (uint32_t)sw::pack_brick_count(ph->dims[1], sw::kPack16.by);
+        a.nb_z = (uint32_t)sw::pack_brick_count(ph->dims[2], sw::kPack16.bz);
     } else {
         const sw::BrickShape sh = field == SW_FIELD_F32 ? sw::kBrick8 : sw::kBrick4;
         a.nb_y = (uint32_t)sw::brick_count(ph->dims[1], sh.lby);

@@ -38,9 +38,26 @@ __global__ void stats_kernel(const uint8_t *mask, const float *field, uint64_t n
     }
 }
 
+// The 16-bit field of a voxel: KIND 1 fp16, 2 bf16 of field_T / half_scale;
+// 4 int16 fixed point rint(field_T / q_scale).
+template <int KIND>
+__device__ uint32_t field16(const float *field, uint64_t v, float inv_scale) {
+    uint32_t h = 0;
+    if (KIND == 1 && field) h = __half_as_ushort(__float2half_rn(field[v] * inv_scale));
+    if (KIND == 2 && field) {
+        const uint32_t bits = __float_as_uint(field[v] * inv_scale);
+        h = (bits + 0x7FFFu + ((bits >> 16) & 1u)) >> 16;          // round to nearest even
+    }
+    if (KIND == 4 && field) {
+        // nearest step, ties to even; |q| <= 32767 up to the rounding of inv_scale
+        const double q = rint((double)field[v] * (double)inv_scale);
+        h = (uint32_t)(int32_t)fmin(32767.0, fmax(-32767.0, q)) & 0xFFFFu;
+    }
+    return h & 0xFFFFu;
+}
+
 // Row-major mask/field -> bricked records (see walk_args.hpp).  KIND 0: Rec8
-// {float field_T, label}; 1: Rec4 fp16; 2: Rec4 bf16; 3: Rec4 label only;
-// 4: Rec4 int16 fixed point.
+// {float field_T, label}; 3: Rec4 label only.
 template <int LX, int LY, int LZ, int KIND>
 __global__ void records_kernel(const uint8_t *mask, const float *field, uint64_t n, uint32_t ny, uint32_t nz,
                                uint64_t nb_y, uint64_t nb_z, float inv_scale, void *out) {
@@ -50,22 +67,25 @@ __global__ void records_kernel(const uint8_t *mask, const float *field, uint64_t
         const uint64_t r = (b << (LX + LY + LZ)) | ((ix & ((1u << LX) - 1)) << (LY + LZ)) |
                            ((iy & ((1u << LY) - 1)) << LZ) | (iz & ((1u << LZ) - 1));
         const uint32_t lab = mask[v];
-        if (KIND == 0) {
-            ((uint2 *)out)[r] = make_uint2(__float_as_uint(field ? field[v] : 0.f), lab);
-        } else {
-            uint32_t h = 0;
-            if (KIND == 1 && field) h = __half_as_ushort(__float2half_rn(field[v] * inv_scale));
-            if (KIND == 2 && field) {
-                const uint32_t bits = __float_as_uint(field[v] * inv_scale);
-                h = (bits + 0x7FFFu + ((bits >> 16) & 1u)) >> 16;      // round to nearest even
-            }
-            if (KIND == 4 && field) {
-                // nearest step, ties to even; |q| <= 32767 up to the rounding of inv_scale
-                const double q = rint((double)field[v] * (double)inv_scale);
-                h = (uint32_t)(int32_t)fmin(32767.0, fmax(-32767.0, q)) & 0xFFFFu;
-            }
-            ((uint32_t *)out)[r] = (h & 0xFFFFu) | (lab << 16);
-        }
+        if (KIND == 0) ((uint2 *)out)[r] = make_uint2(__float_as_uint(field ? field[v] : 0.f), lab);
+        else ((uint32_t *)out)[r] = lab << 16;
+    }
+}
+
+// Row-major mask/field -> packed 3-byte 16-bit records (kPack16).
+template <int KIND>
+__global__ void packed16_records_kernel(const uint8_t *mask, const float *field, uint64_t n, uint32_t ny,
+                                        uint32_t nz, uint64_t nb_y, uint64_t nb_z, float inv_scale, uint8_t *out) {
+    constexpr PackShape S = kPack16;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t iz = v % nz, iy = (v / nz) % ny, ix = v / ((uint64_t)nz * ny);
+        const uint64_t b = ((ix / S.bx) * nb_y + iy / S.by) * nb_z + iz / S.bz;
+        const uint32_t j = (uint32_t)(((ix % S.bx) * S.by + iy % S.by) * S.bz + iz % S.bz);
+        uint8_t *r = out + b * 128 + S.rec_bytes * j;
+        const uint32_t h = field16<KIND>(field, v, inv_scale);
+        r[0] = mask[v];
+        r[1] = (uint8_t)h;
+        r[2] = (uint8_t)(h >> 8);
     }
 }
 
@@ -118,20 +138,22 @@ hipError_t launch_records(const uint8_t *mask, const float *field, uint64_t nx, 
         const uint64_t nby = brick_count(ny, kBrick8.lby), nbz = brick_count(nz, kBrick8.lbz);
         hipLaunchKernelGGL((records_kernel<kBrick8.lbx, kBrick8.lby, kBrick8.lbz, 0>), g, b, 0, s, mask, field, n,
                            (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
-    } else {
+    } else if (kind == 3) {
         const uint64_t nby = brick_count(ny, kBrick4.lby), nbz = brick_count(nz, kBrick4.lbz);
+        hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 3>), g, b, 0, s, mask, field, n,
+                           (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
+    } else {
+        const uint64_t nby = pack_brick_count(ny, kPack16.by), nbz = pack_brick_count(nz, kPack16.bz);
+        uint8_t *o = (uint8_t *)out;
         if (kind == 1)
-            hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 1>), g, b, 0, s, mask, field, n,
-                               (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
+            hipLaunchKernelGGL((packed16_records_kernel<1>), g, b, 0, s, mask, field, n, (uint32_t)ny, (uint32_t)nz,
+                               nby, nbz, inv_scale, o);
         else if (kind == 2)
-            hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 2>), g, b, 0, s, mask, field, n,
-                               (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
-        else if (kind == 4)
-            hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 4>), g, b, 0, s, mask, field, n,
-                               (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
+            hipLaunchKernelGGL((packed16_records_kernel<2>), g, b, 0, s, mask, field, n, (uint32_t)ny, (uint32_t)nz,
+                               nby, nbz, inv_scale, o);
         else
-            hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 3>), g, b, 0, s, mask, field, n,
-                               (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
+            hipLaunchKernelGGL((packed16_records_kernel<4>), g, b, 0, s, mask, field, n, (uint32_t)ny, (uint32_t)nz,
+                               nby, nbz, inv_scale, o);
     }
     return hipGetLastError();
 }

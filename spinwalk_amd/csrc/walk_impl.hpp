@@ -177,31 +177,34 @@ struct RecNone {                         // label only: {0, uint8 label, pad}
     __device__ static uint32_t label(T r) { return r >> 16; }
     template <class R> __device__ static R field(const WalkArgs &, T) { return (R)0; }
 };
-struct RecF16 {                          // {fp16 field_T/half_scale, uint8 label, pad}
-    static constexpr BrickShape shape = kBrick4;
-    static constexpr bool kPacked = false;
-    using T = uint32_t;
-    __device__ static uint32_t label(T r) { return r >> 16; }
-    template <class R> __device__ static R field(const WalkArgs &a, T r) {
-        return (R)(__half2float(__ushort_as_half((unsigned short)(r & 0xFFFFu))) * a.h2d);
+// The 16-bit fields: packed 3-byte records {uint8 label, 16-bit field} in
+// bricks of 3 x 3 x 4 voxels (kPack16).  A record index is the byte offset of
+// the record; one 4-byte load there (unaligned, inside the line) brings the
+// label (byte 0) and the field (bytes 1-2).
+typedef uint32_t __attribute__((aligned(1))) u32_unaligned;
+struct Rec16 {
+    static constexpr bool kPacked = true;
+    static constexpr PackShape pack = kPack16;
+    using T = uint32_t;                  // the 4 bytes loaded
+    __device__ static uint32_t label(T r) { return r & 0xFFu; }
+    __device__ static uint32_t bits(T r) { return (r >> 8) & 0xFFFFu; }
+    template <bool IDX32, class I> __device__ static T load(const void *base, I off) {
+        return gload<u32_unaligned>(base, off);
     }
 };
-struct RecQ16 {                          // {int16 rint(field_T/q_scale), uint8 label, pad}
-    static constexpr BrickShape shape = kBrick4;
-    static constexpr bool kPacked = false;
-    using T = uint32_t;
-    __device__ static uint32_t label(T r) { return r >> 16; }
+struct RecF16 : Rec16 {                  // fp16 field_T / half_scale
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
-        return (R)((float)(int32_t)(int16_t)(uint16_t)(r & 0xFFFFu) * a.h2d);
+        return (R)(__half2float(__ushort_as_half((unsigned short)bits(r))) * a.h2d);
     }
 };
-struct RecBF16 {                         // {bf16 field_T/half_scale, uint8 label, pad}
-    static constexpr BrickShape shape = kBrick4;
-    static constexpr bool kPacked = false;
-    using T = uint32_t;
-    __device__ static uint32_t label(T r) { return r >> 16; }
+struct RecQ16 : Rec16 {                  // int16 rint(field_T / q_scale)
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
-        return (R)(__uint_as_float(r << 16) * a.h2d);
+        return (R)((float)(int32_t)(int16_t)(uint16_t)bits(r) * a.h2d);
+    }
+};
+struct RecBF16 : Rec16 {                 // bf16 field_T / half_scale
+    template <class R> __device__ static R field(const WalkArgs &a, T r) {
+        return (R)(__uint_as_float(bits(r) << 16) * a.h2d);
     }
 };
 

@@ -41,10 +41,10 @@ hipError_t launch_gather_probe(const void *buf, uint64_t bytes, uint32_t n_lanes
 // uint32 bit patterns; stats must be zeroed by the caller).
 hipError_t launch_phantom_stats(const uint8_t *mask, const float *field, uint64_t n,
                                 uint32_t *stats, hipStream_t s);
-// Row-major mask/field -> bricked voxel records: kind 0 Rec8 f32, 1 Rec4 fp16,
-// 2 Rec4 bf16, 3 Rec4 label only, 4 Rec4 int16 fixed point.  inv_scale divides
-// the field before the 16-bit kinds round it.  `out` must be zero-filled for
-// padding.
+// Row-major mask/field -> bricked voxel records: kind 0 Rec8 f32, 3 Rec4 label
+// only; kinds 1 fp16, 2 bf16 and 4 int16 fixed point as packed 3-byte records
+// (kPack16).  inv_scale divides the field before the 16-bit kinds round it.
+// `out` must be zero-filled for padding.
 hipError_t launch_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz, int kind,
                           float inv_scale, void *out, hipStream_t s);
 // Row-major mask/field -> the packed float32 bricks (kPackF32) of
