@@ -62,6 +62,12 @@ constexpr PackShape kPackF32 = {2, 3, 4, 24, 5};
 // byte 107).  A spin leaves a 3 x 3 x 4 brick 8 % less often than a 4 x 4 x 2
 // one (the sums of the inverse sides, 0.917 against 1.0).
 constexpr PackShape kPack16 = {3, 3, 4, 36, 3};
+// A brick fills its slots, and the load at the last record (8 bytes for the
+// float32 records, 4 for the 16-bit ones) ends inside the 128-byte line.
+static_assert(kPackF32.nvox == kPackF32.bx * kPackF32.by * kPackF32.bz &&
+              (kPackF32.nvox - 1) * kPackF32.rec_bytes + 8 <= 128, "float32 records leave their line");
+static_assert(kPack16.nvox == kPack16.bx * kPack16.by * kPack16.bz &&
+              (kPack16.nvox - 1) * kPack16.rec_bytes + 4 <= 128, "16-bit records leave their line");
 SW_HD_INLINE uint64_t pack_brick_count(uint64_t n, int b) { return (n + b - 1) / b; }
 
 SW_HD_INLINE uint64_t brick_count(uint64_t n, int lb) { return (n + (1ull << lb) - 1) >> lb; }
