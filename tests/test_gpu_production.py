@@ -217,8 +217,8 @@ def test_production_gathers_every_voxel_exactly(engine, dims):
     """With zero diffusivity nothing moves and no draw is taken, so the
     production walk is deterministic: one spin at the centre of every voxel of a
     phantom whose sides are not multiples of the record bricks (2 x 3 x 4 for the
-    production walk's float32 structure-of-arrays records, 4 x 2 x 2 for the
-    reference layout) must see that voxel's label and field, as the oracle does:
+    production walk's packed float32 records, 4 x 2 x 2 for the reference
+    layout) must see that voxel's label and field, as the oracle does:
     labels equal, M within 1e-5 (a GRE of 20 steps, each adding the voxel's
     field as a phase)."""
     m, f = cases.blocky_phantom(dims, n_labels=3, seed=7, block=2)
@@ -236,6 +236,54 @@ def test_production_gathers_every_voxel_exactly(engine, dims):
         np.testing.assert_array_equal(g["T"], o["T"])
         mag = np.linalg.norm(o["M1"], axis=-1, keepdims=True)
         assert (np.abs(g["M1"] - o["M1"]) <= 1e-5 * mag + 1e-7).all()
+
+
+def _dequantized(f, fmt):
+    """The float32 field a 16-bit record holds for each voxel, as the engine
+    quantizes it (phantom.hip): float16 / bfloat16 of f / s, s = 2^(e - 15)
+    with max|f| = m 2^e, m in [0.5, 1); int16 rint(f / q), q = max|f| / 32767."""
+    f = np.asarray(f, np.float32)
+    fmax = np.float32(np.abs(f).max())
+    if fmt == E.SW_FIELD_Q16:
+        q = np.float32(np.float64(fmax) / 32767.0)
+        inv = np.float32(1.0) / q
+        k = np.clip(np.rint(f.astype(np.float64) * np.float64(inv)), -32767, 32767)
+        return (k.astype(np.float32) * q).astype(np.float32)
+    _, ex = np.frexp(fmax)
+    s = np.float32(2.0 ** (int(ex) - 15))
+    x = (f * (np.float32(1.0) / s)).astype(np.float32)
+    if fmt == E.SW_FIELD_F16:
+        h = x.astype(np.float16).astype(np.float32)
+    else:
+        b = x.view(np.uint32).astype(np.uint64)
+        h = (((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
+    return (h * s).astype(np.float32)
+
+
+@pytest.mark.parametrize("dims", [(13, 7, 10), (8, 9, 12)])
+@pytest.mark.parametrize("fmt", [E.SW_FIELD_F16, E.SW_FIELD_BF16, E.SW_FIELD_Q16])
+def test_production_gathers_16bit_records_exactly(engine, dims, fmt):
+    """The 16-bit records (packed 3-byte records in 3 x 3 x 4 bricks) on sides
+    that are not multiples of the bricks: with zero diffusivity every spin stays
+    at the centre of its voxel, so it must see that voxel's label, and its
+    magnetisation must be the oracle's for the field the record holds (the
+    float32 field quantized as the engine quantizes it): labels equal, M within
+    1e-5, in the Kahan phase the 16-bit fields are meant for."""
+    m, f = cases.blocky_phantom(dims, n_labels=3, seed=8, block=2)
+    f = (f * 30).astype(np.float32)
+    cfg = cases.gre(3, TE=1000, dt=50, DIFFUSIVITY=[0.0, 0.0, 0.0], T1=[1000.0, 900.0, 800.0], T2=[60.0, 50.0, 40.0])
+    fov_um = 2.0 * np.asarray(dims, float)
+    idx = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), -1).reshape(-1, 3)
+    xyz0 = ((idx + 0.5) * 2e-6).astype(np.float32)
+    case = cases.make_case(cfg, m, f, fov_um, n_spins=len(xyz0), xyz0=xyz0)
+    g = run(engine, case, precision=E.SW_PREC_F32_KAHAN, field_format=fmt, formats=(E.SW_FIELD_F32, fmt))
+    ref = cases.make_case(cfg, m, _dequantized(f, fmt), fov_um, n_spins=len(xyz0), xyz0=xyz0)
+    o = orc.sim(cases.to_oracle(ref))
+    assert g["stats"]["n_lost"] == 0 and g["stats"]["n_attempts"] == g["stats"]["n_steps"]
+    np.testing.assert_array_equal(g["T"][:, 0], m.reshape(-1))
+    np.testing.assert_array_equal(g["T"], o["T"])
+    mag = np.linalg.norm(o["M1"], axis=-1, keepdims=True)
+    assert (np.abs(g["M1"] - o["M1"]) <= 1e-5 * mag + 1e-7).all()
 
 
 def test_pgse_discrete_stejskal_tanner(engine):
