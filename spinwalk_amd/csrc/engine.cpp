@@ -75,7 +75,8 @@ struct sw_phantom {
     bool has_field = false;
     // bricked voxel records gathered by the walk (walk_args.hpp)
     void *rec[5] = {};               // by SW_FIELD_*: none (label only), f32, f16, bf16, q16
-    void *rec_packed = nullptr;      // float32 packed 5-byte records (production walks)
+    void *rec_packed = nullptr;      // float32 packed 5-byte records (production batches)
+    void *rec_soa = nullptr;         // float32 structure-of-arrays records, same bricks (single production walks)
 };
 
 struct sw_sequence {
@@ -372,14 +373,15 @@ SW_API int sw_replay_fill(uint64_t seed, uint32_t spin_begin, uint32_t n, float 
 // ---------------------------------------------------------------------------
 // phantom
 // ---------------------------------------------------------------------------
-// HBM a phantom takes: mask, float32 field and the bricked records (8 bytes a
-// voxel for float32, 4 for the others and for the label-only records).
+// HBM a phantom takes: mask, float32 field and the bricked records (for
+// float32 8 bytes a voxel and twice 5.33 for the production copies, 4 for the
+// label-only records, 3.56 for the 16-bit ones, rounded up).
 static size_t phantom_bytes(uint64_t n, bool has_field, uint32_t formats) {
     size_t need = n;
     if (!has_field) return need + n * 4;
     need += n * 4;
     for (int f = SW_FIELD_F32; f <= SW_FIELD_Q16; f++)
-        if (formats & (1u << f)) need += n * (f == SW_FIELD_F32 ? 8 + 6 : 4);   // f32: 8-byte and packed records
+        if (formats & (1u << f)) need += n * (f == SW_FIELD_F32 ? 8 + 6 + 6 : 4);   // f32: 8-byte, packed and SoA records
     return need;
 }
 
@@ -425,13 +427,16 @@ static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
         const int kind = f == SW_FIELD_F32 ? 0 : f == SW_FIELD_F16 ? 1 : f == SW_FIELD_BF16 ? 2 : f == SW_FIELD_Q16 ? 4 : 3;
         SW_HIP(e, sw::launch_records(ph->mask, ph->field, nx, ny, nz, kind,
                                      1.0f / (f == SW_FIELD_Q16 ? ph->q_scale : ph->half_scale), ph->rec[f], s));
-        if (f == SW_FIELD_F32) {         // the production walk's packed 5-byte copy
+        if (f == SW_FIELD_F32) {         // the production walks' copies: packed for batches, SoA for single walks
             const size_t bytes = (size_t)(sw::pack_brick_count(nx, sw::kPackF32.bx) *
                                           sw::pack_brick_count(ny, sw::kPackF32.by) *
                                           sw::pack_brick_count(nz, sw::kPackF32.bz)) * 128;
-            SW_HIP(e, hipMalloc(&ph->rec_packed, bytes));
-            SW_HIP(e, hipMemsetAsync(ph->rec_packed, 0, bytes, s));
-            SW_HIP(e, sw::launch_packed_records(ph->mask, ph->field, nx, ny, nz, ph->rec_packed, s));
+            for (int soa = 0; soa < 2; soa++) {
+                void *&r = soa ? ph->rec_soa : ph->rec_packed;
+                SW_HIP(e, hipMalloc(&r, bytes));
+                SW_HIP(e, hipMemsetAsync(r, 0, bytes, s));
+                SW_HIP(e, sw::launch_f32_records(ph->mask, ph->field, nx, ny, nz, soa != 0, r, s));
+            }
         }
     }
     SW_HIP(e, hipStreamSynchronize(s));
@@ -484,6 +489,7 @@ SW_API void sw_phantom_destroy(sw_phantom *ph) {
     dev_free(ph->field);
     for (void *r : ph->rec) dev_free(r);
     dev_free(ph->rec_packed);
+    dev_free(ph->rec_soa);
     delete ph;
 }
 
@@ -795,9 +801,10 @@ int fill_walk_args(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, con
     std::memset(&a, 0, sizeof a);
     a.rec = ph->rec[field];
     if (field == SW_FIELD_F32 && o->rng == SW_RNG_PHILOX) {
-        // production walks gather the float32 field from the packed 5-byte records
-        if (!ph->rec_packed) return fail(e, SW_E_INVALID, "the phantom has no packed float32 records");
-        a.rec = ph->rec_packed;
+        // a single production walk gathers the float32 field from the
+        // structure-of-arrays bricks (a batch swaps in the packed records)
+        if (!ph->rec_soa || !ph->rec_packed) return fail(e, SW_E_INVALID, "the phantom has no production float32 records");
+        a.rec = ph->rec_soa;
         a.nb_y = (uint32_t)sw::pack_brick_count(ph->dims[1], sw::kPackF32.by);
         a.nb_z = (uint32_t)sw::pack_brick_count(ph->dims[2], sw::kPackF32.bz);
     } else if (field == SW_FIELD_F16 || field == SW_FIELD_BF16 || field == SW_FIELD_Q16) {
@@ -1292,6 +1299,8 @@ SW_API int sw_batch_create(sw_engine *e, const sw_batch_item *items, uint32_t n_
         int rc = fill_walk_args(e, it.phantom, it.sequence, &it.params, &it.opts, it.d_M0, it.d_XYZ0, it.d_M1,
                                 it.d_XYZ1, it.d_T, args[i], &field);
         if (rc) { e->err = "tile " + std::to_string(i) + ": " + e->err; return rc; }
+        // batches gather the float32 field from the packed records (same bricks)
+        if (field == SW_FIELD_F32 && it.opts.rng == SW_RNG_PHILOX) args[i].rec = it.phantom->rec_packed;
         if (i == 0) b->field = field;
         else if (field != b->field)
             return fail(e, SW_E_INVALID, "tile " + std::to_string(i) + ": every tile of a batch gathers the same field format");

@@ -89,32 +89,44 @@ __global__ void packed16_records_kernel(const uint8_t *mask, const float *field,
     }
 }
 
-// Row-major mask/field -> packed float32 bricks (kPackF32).
-__global__ void packed_records_kernel(const uint8_t *mask, const float *field, uint64_t n, uint32_t ny, uint32_t nz,
-                                      uint64_t nb_y, uint64_t nb_z, uint8_t *out) {
-    constexpr PackShape S = kPackF32;
+// Row-major mask/field -> float32 bricks of 2 x 3 x 4 voxels: packed 5-byte
+// records (kPackF32) or structure-of-arrays records (kSoaF32).
+template <bool SOA>
+__global__ void f32_records_kernel(const uint8_t *mask, const float *field, uint64_t n, uint32_t ny, uint32_t nz,
+                                   uint64_t nb_y, uint64_t nb_z, uint8_t *out) {
+    constexpr PackShape S = SOA ? kSoaF32 : kPackF32;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t iz = v % nz, iy = (v / nz) % ny, ix = v / ((uint64_t)nz * ny);
         const uint64_t b = ((ix / S.bx) * nb_y + iy / S.by) * nb_z + iz / S.bz;
         const uint32_t j = (uint32_t)(((ix % S.bx) * S.by + iy % S.by) * S.bz + iz % S.bz);
-        uint8_t *r = out + b * 128 + S.rec_bytes * j;
         const uint32_t f = __float_as_uint(field ? field[v] : 0.f);
-        r[0] = mask[v];
-        r[1] = (uint8_t)f; r[2] = (uint8_t)(f >> 8); r[3] = (uint8_t)(f >> 16); r[4] = (uint8_t)(f >> 24);
+        if constexpr (SOA) {
+            uint8_t *brick = out + b * 128;
+            brick[j] = mask[v];
+            *(uint32_t *)(brick + S.nvox + 4 * j) = f;
+        } else {
+            uint8_t *r = out + b * 128 + S.rec_bytes * j;
+            r[0] = mask[v];
+            r[1] = (uint8_t)f; r[2] = (uint8_t)(f >> 8); r[3] = (uint8_t)(f >> 16); r[4] = (uint8_t)(f >> 24);
+        }
     }
 }
 
 }  // namespace
 
-hipError_t launch_packed_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz,
-                              void *out, hipStream_t s) {
+hipError_t launch_f32_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz,
+                              bool soa, void *out, hipStream_t s) {
     const uint64_t n = nx * ny * nz;
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 16384) blocks = 16384;
     const uint64_t nby = pack_brick_count(ny, kPackF32.by), nbz = pack_brick_count(nz, kPackF32.bz);
-    hipLaunchKernelGGL(packed_records_kernel, dim3((unsigned)blocks), dim3(256), 0, s, mask, field, n, (uint32_t)ny,
-                       (uint32_t)nz, nby, nbz, (uint8_t *)out);
+    if (soa)
+        hipLaunchKernelGGL(f32_records_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, mask, field, n,
+                           (uint32_t)ny, (uint32_t)nz, nby, nbz, (uint8_t *)out);
+    else
+        hipLaunchKernelGGL(f32_records_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, mask, field, n,
+                           (uint32_t)ny, (uint32_t)nz, nby, nbz, (uint8_t *)out);
     return hipGetLastError();
 }
 

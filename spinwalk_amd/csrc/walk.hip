@@ -14,7 +14,7 @@ hipError_t launch_walk(const WalkArgs &a, int rng, int field, int prec, hipStrea
 WalkMode walk_mode(const WalkArgs &a, int field, int rng) {
     switch (field) {
     case 0: return mode_of<RecNone>(a);
-    case 1: return rng == 1 ? mode_of<RecF32>(a) : mode_of<RecF32P>(a);
+    case 1: return rng == 1 ? mode_of<RecF32>(a) : mode_of<RecF32P>(a);   // RecF32S: the same bricks
     case 2: return mode_of<RecF16>(a);
     case 4: return mode_of<RecQ16>(a);
     default: return mode_of<RecBF16>(a);

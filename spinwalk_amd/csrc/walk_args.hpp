@@ -56,6 +56,12 @@ constexpr BrickShape kBrick4 = {2, 2, 1};
 // together.  Bricks are row-major, as for BrickShape.
 struct PackShape { int bx, by, bz, nvox, rec_bytes; };
 constexpr PackShape kPackF32 = {2, 3, 4, 24, 5};
+// The same bricks as structure-of-arrays records: the 24 labels at bytes 0-23
+// of the line, then the 24 float32 fields, aligned, at bytes 24 + 4 j.  Two
+// aligned loads a gather (label at byte j, field at byte 24 + 4 j) instead of
+// one unaligned one.  As a PackShape its records are the 1-byte labels, so a
+// record's byte offset is its label's.
+constexpr PackShape kSoaF32 = {2, 3, 4, 24, 1};
 // The 16-bit fields (float16, bfloat16, int16) pack the same way: 3-byte
 // records {label, 16-bit field} in bricks of 3 x 3 x 4 voxels, 36 to a line
 // (bytes 108-127 padding), read with one 4-byte load at byte 3 j (it ends by
@@ -66,6 +72,9 @@ constexpr PackShape kPack16 = {3, 3, 4, 36, 3};
 // float32 records, 4 for the 16-bit ones) ends inside the 128-byte line.
 static_assert(kPackF32.nvox == kPackF32.bx * kPackF32.by * kPackF32.bz &&
               (kPackF32.nvox - 1) * kPackF32.rec_bytes + 8 <= 128, "float32 records leave their line");
+static_assert(kSoaF32.bx == kPackF32.bx && kSoaF32.by == kPackF32.by && kSoaF32.bz == kPackF32.bz &&
+              kSoaF32.nvox == kPackF32.nvox && kSoaF32.nvox * (1 + 4) <= 128,
+              "structure-of-arrays float32 bricks differ from the packed ones or leave their line");
 static_assert(kPack16.nvox == kPack16.bx * kPack16.by * kPack16.bz &&
               (kPack16.nvox - 1) * kPack16.rec_bytes + 4 <= 128, "16-bit records leave their line");
 SW_HD_INLINE uint64_t pack_brick_count(uint64_t n, int b) { return (n + b - 1) / b; }
@@ -95,7 +104,7 @@ struct WalkView {
 
 struct WalkArgs {
     // ---- phantom -----------------------------------------------------------
-    const void     *rec;          // bricked voxel records (Rec8, Rec4 or the packed float32 bricks)
+    const void     *rec;          // bricked voxel records (Rec8, Rec4, packed or structure-of-arrays)
     uint32_t        nb_y, nb_z;   // bricks along y and z
     float           t2d;          // float T -> deg/step factor (monte_carlo.cu:241)
     float           h2d;          // half -> deg/step factor = half_scale * t2d

@@ -229,6 +229,25 @@ struct RecF32P {
     }
 };
 
+// The single production walk's float32 records: the same bricks as
+// structure-of-arrays (kSoaF32).  A record index is the byte offset of the
+// label; the field sits 24 + 3 j bytes further, j = the slot in the brick =
+// offset mod 128.  Two aligned loads from one line.
+struct RecF32S {
+    static constexpr bool kPacked = true;
+    static constexpr PackShape pack = kSoaF32;
+    using T = uint2;                     // {field bits, label}, as RecF32
+    __device__ static uint32_t label(T r) { return r.y; }
+    template <class R> __device__ static R field(const WalkArgs &a, T r) { return RecF32::field<R>(a, r); }
+    template <bool IDX32, class I> __device__ static T load(const void *base, I off) {
+        const I foff = off + (I)3 * (off & (I)127);                       // field - nvox, as an offset
+        T r;
+        r.y = gload<uint8_t>(base, off);
+        r.x = gload<uint32_t>((const char *)base + kSoaF32.nvox, foff);
+        return r;
+    }
+};
+
 template <class Rec, bool IDX32, class I>
 __device__ __forceinline__ typename Rec::T rec_load(const void *base, I off) {
     if constexpr (Rec::kPacked) return Rec::template load<IDX32>(base, off);
@@ -1546,8 +1565,13 @@ hipError_t launch_batch_fp(const void *items, const BatchChunk *chunks, uint32_t
 }
 
 // The float32 records a walk gathers: the reference layout for replayed draws
-// (held bit for bit to the oracle), the packed 5-byte records in production.
-template <class Rng> using RecF32Of = typename std::conditional<Rng::kReplay, RecF32, RecF32P>::type;
+// (held bit for bit to the oracle); in production, the packed 5-byte records
+// for batches, which are bound by VALU issue and gain from one load a gather,
+// and the structure-of-arrays bricks for single walks, which wait on HBM and
+// run faster with two aligned loads than with one unaligned one.  Both hold
+// the same bricks, so both give the same results bit for bit.
+template <class Rng> using RecF32Batch = typename std::conditional<Rng::kReplay, RecF32, RecF32P>::type;
+template <class Rng> using RecF32Single = typename std::conditional<Rng::kReplay, RecF32, RecF32S>::type;
 
 template <class Rng>
 hipError_t launch_batch_rng(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
@@ -1555,16 +1579,16 @@ hipError_t launch_batch_rng(const void *items, const BatchChunk *chunks, uint32_
                             hipStream_t s) {
     if (prec == 2) {
         if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF64>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32Of<Rng>, PhaseF64>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32Batch<Rng>, PhaseF64>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
     } else if (prec == 1) {
         if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32Of<Rng>, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32Batch<Rng>, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
         if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
         if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
         if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
     } else if (prec == 0) {
         if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32Of<Rng>, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32Batch<Rng>, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
         if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
         if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
         if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
@@ -1579,16 +1603,16 @@ hipError_t launch_rng(const WalkArgs &a, int field, int prec, hipStream_t s) {
     const dim3 block(256), grid((nb + 7) / 8 * 8);
     if (prec == 2) {
         if (field == 0) return launch_fp<Rng, RecNone, PhaseF64>(a, s, grid, block);
-        if (field == 1) return launch_fp<Rng, RecF32Of<Rng>, PhaseF64>(a, s, grid, block);
+        if (field == 1) return launch_fp<Rng, RecF32Single<Rng>, PhaseF64>(a, s, grid, block);
     } else if (prec == 1) {
         if (field == 0) return launch_fp<Rng, RecNone, PhaseKahan>(a, s, grid, block);
-        if (field == 1) return launch_fp<Rng, RecF32Of<Rng>, PhaseKahan>(a, s, grid, block);
+        if (field == 1) return launch_fp<Rng, RecF32Single<Rng>, PhaseKahan>(a, s, grid, block);
         if (field == 2) return launch_fp<Rng, RecF16, PhaseKahan>(a, s, grid, block);
         if (field == 3) return launch_fp<Rng, RecBF16, PhaseKahan>(a, s, grid, block);
         if (field == 4) return launch_fp<Rng, RecQ16, PhaseKahan>(a, s, grid, block);
     } else if (prec == 0) {
         if (field == 0) return launch_fp<Rng, RecNone, PhaseF32>(a, s, grid, block);
-        if (field == 1) return launch_fp<Rng, RecF32Of<Rng>, PhaseF32>(a, s, grid, block);
+        if (field == 1) return launch_fp<Rng, RecF32Single<Rng>, PhaseF32>(a, s, grid, block);
         if (field == 2) return launch_fp<Rng, RecF16, PhaseF32>(a, s, grid, block);
         if (field == 3) return launch_fp<Rng, RecBF16, PhaseF32>(a, s, grid, block);
         if (field == 4) return launch_fp<Rng, RecQ16, PhaseF32>(a, s, grid, block);

@@ -47,10 +47,11 @@ hipError_t launch_phantom_stats(const uint8_t *mask, const float *field, uint64_
 // `out` must be zero-filled for padding.
 hipError_t launch_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz, int kind,
                           float inv_scale, void *out, hipStream_t s);
-// Row-major mask/field -> the packed float32 bricks (kPackF32) of
-// the production walk; `out` must be zero-filled for padding.
-hipError_t launch_packed_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz,
-                              void *out, hipStream_t s);
+// Row-major mask/field -> the float32 bricks of production walks: the packed
+// 5-byte records (kPackF32) batches gather, or with soa the structure-of-arrays
+// records (kSoaF32) single walks gather; `out` must be zero-filled for padding.
+hipError_t launch_f32_records(const uint8_t *mask, const float *field, uint64_t nx, uint64_t ny, uint64_t nz,
+                              bool soa, void *out, hipStream_t s);
 // FP64 echo reduction into out[(te*n_sub+label)*3 + {0,1,2}]; partial is scratch
 // of reduce_partial_len(n_spins, n_te, n_sub) doubles.
 uint64_t reduce_partial_len(uint32_t n_spins, uint32_t n_te, uint32_t n_sub);
