@@ -415,8 +415,8 @@ static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
         if (!want) continue;
         size_t bytes;
         if (f == SW_FIELD_F16 || f == SW_FIELD_BF16 || f == SW_FIELD_Q16) {   // packed 3-byte records
-            bytes = (size_t)(sw::pack_brick_count(nx, sw::kPack16.bx) * sw::pack_brick_count(ny, sw::kPack16.by) *
-                             sw::pack_brick_count(nz, sw::kPack16.bz)) * 128;
+            bytes = (size_t)(sw::pack_brick_count(nx, sw::kPack16.bx) * sw::pack_pitch(ny, sw::kPack16.by) *
+                             sw::pack_pitch(nz, sw::kPack16.bz)) * 128;
         } else {
             const sw::BrickShape sh = f == SW_FIELD_F32 ? sw::kBrick8 : sw::kBrick4;
             bytes = ((size_t)(sw::brick_count(nx, sh.lbx) * sw::brick_count(ny, sh.lby) * sw::brick_count(nz, sh.lbz))
@@ -429,8 +429,8 @@ static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
                                      1.0f / (f == SW_FIELD_Q16 ? ph->q_scale : ph->half_scale), ph->rec[f], s));
         if (f == SW_FIELD_F32) {         // the production walks' copies: packed for batches, SoA for single walks
             const size_t bytes = (size_t)(sw::pack_brick_count(nx, sw::kPackF32.bx) *
-                                          sw::pack_brick_count(ny, sw::kPackF32.by) *
-                                          sw::pack_brick_count(nz, sw::kPackF32.bz)) * 128;
+                                          sw::pack_pitch(ny, sw::kPackF32.by) *
+                                          sw::pack_pitch(nz, sw::kPackF32.bz)) * 128;
             for (int soa = 0; soa < 2; soa++) {
                 void *&r = soa ? ph->rec_soa : ph->rec_packed;
                 SW_HIP(e, hipMalloc(&r, bytes));
@@ -805,11 +805,11 @@ int fill_walk_args(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, con
         // structure-of-arrays bricks (a batch swaps in the packed records)
         if (!ph->rec_soa || !ph->rec_packed) return fail(e, SW_E_INVALID, "the phantom has no production float32 records");
         a.rec = ph->rec_soa;
-        a.nb_y = (uint32_t)sw::pack_brick_count(ph->dims[1], sw::kPackF32.by);
-        a.nb_z = (uint32_t)sw::pack_brick_count(ph->dims[2], sw::kPackF32.bz);
+        a.nb_y = (uint32_t)sw::pack_pitch(ph->dims[1], sw::kPackF32.by);
+        a.nb_z = (uint32_t)sw::pack_pitch(ph->dims[2], sw::kPackF32.bz);
     } else if (field == SW_FIELD_F16 || field == SW_FIELD_BF16 || field == SW_FIELD_Q16) {
-        a.nb_y = (uint32_t)sw::pack_brick_count(ph->dims[1], sw::kPack16.by);
-        a.nb_z = (uint32_t)sw::pack_brick_count(ph->dims[2], sw::kPack16.bz);
+        a.nb_y = (uint32_t)sw::pack_pitch(ph->dims[1], sw::kPack16.by);
+        a.nb_z = (uint32_t)sw::pack_pitch(ph->dims[2], sw::kPack16.bz);
     } else {
         const sw::BrickShape sh = field == SW_FIELD_F32 ? sw::kBrick8 : sw::kBrick4;
         a.nb_y = (uint32_t)sw::brick_count(ph->dims[1], sh.lby);

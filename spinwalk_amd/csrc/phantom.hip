@@ -120,7 +120,7 @@ hipError_t launch_f32_records(const uint8_t *mask, const float *field, uint64_t 
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 16384) blocks = 16384;
-    const uint64_t nby = pack_brick_count(ny, kPackF32.by), nbz = pack_brick_count(nz, kPackF32.bz);
+    const uint64_t nby = pack_pitch(ny, kPackF32.by), nbz = pack_pitch(nz, kPackF32.bz);
     if (soa)
         hipLaunchKernelGGL(f32_records_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, mask, field, n,
                            (uint32_t)ny, (uint32_t)nz, nby, nbz, (uint8_t *)out);
@@ -155,7 +155,7 @@ hipError_t launch_records(const uint8_t *mask, const float *field, uint64_t nx, 
         hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 3>), g, b, 0, s, mask, field, n,
                            (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
     } else {
-        const uint64_t nby = pack_brick_count(ny, kPack16.by), nbz = pack_brick_count(nz, kPack16.bz);
+        const uint64_t nby = pack_pitch(ny, kPack16.by), nbz = pack_pitch(nz, kPack16.bz);
         uint8_t *o = (uint8_t *)out;
         if (kind == 1)
             hipLaunchKernelGGL((packed16_records_kernel<1>), g, b, 0, s, mask, field, n, (uint32_t)ny, (uint32_t)nz,

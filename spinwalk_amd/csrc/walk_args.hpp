@@ -78,6 +78,11 @@ static_assert(kSoaF32.bx == kPackF32.bx && kSoaF32.by == kPackF32.by && kSoaF32.
 static_assert(kPack16.nvox == kPack16.bx * kPack16.by * kPack16.bz &&
               (kPack16.nvox - 1) * kPack16.rec_bytes + 4 <= 128, "16-bit records leave their line");
 SW_HD_INLINE uint64_t pack_brick_count(uint64_t n, int b) { return (n + b - 1) / b; }
+// Bricks along y and z as laid out: the count rounded up to an odd number, so
+// that bricks adjacent in y (nb_z lines apart) or in x (nb_y nb_z lines apart)
+// are an odd number of lines apart, not a multiple of a large power of two
+// (the bricks of the padding slots are never gathered).
+SW_HD_INLINE uint64_t pack_pitch(uint64_t n, int b) { return pack_brick_count(n, b) | 1; }
 
 SW_HD_INLINE uint64_t brick_count(uint64_t n, int lb) { return (n + (1ull << lb) - 1) >> lb; }
 
