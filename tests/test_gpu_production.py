@@ -212,13 +212,15 @@ def test_production_boundaries_match_the_reference(engine):
             assert edge(x[:, 0]) < 0.05 and edge(xo[:, 0]) < 0.05
 
 
-@pytest.mark.parametrize("dims", [(13, 7, 10), (8, 9, 12)])
+@pytest.mark.parametrize("dims", [(13, 7, 10), (8, 9, 12), (10, 12, 8)])
 def test_production_gathers_every_voxel_exactly(engine, dims):
     """With zero diffusivity nothing moves and no draw is taken, so the
     production walk is deterministic: one spin at the centre of every voxel of a
     phantom whose sides are not multiples of the record bricks (2 x 3 x 4 for the
-    production walk's packed float32 records, 4 x 2 x 2 for the reference
-    layout) must see that voxel's label and field, as the oracle does:
+    production walk's float32 records, 4 x 2 x 2 for the reference layout),
+    or whose brick counts along y and z are even, which the production layout
+    pads to odd counts (10 x 12 x 8: 4 and 2 bricks laid out as 5 and 3),
+    must see that voxel's label and field, as the oracle does:
     labels equal, M within 1e-5 (a GRE of 20 steps, each adding the voxel's
     field as a phase)."""
     m, f = cases.blocky_phantom(dims, n_labels=3, seed=7, block=2)
@@ -260,11 +262,12 @@ def _dequantized(f, fmt):
     return (h * s).astype(np.float32)
 
 
-@pytest.mark.parametrize("dims", [(13, 7, 10), (8, 9, 12)])
+@pytest.mark.parametrize("dims", [(13, 7, 10), (8, 9, 12), (10, 12, 8)])
 @pytest.mark.parametrize("fmt", [E.SW_FIELD_F16, E.SW_FIELD_BF16, E.SW_FIELD_Q16])
 def test_production_gathers_16bit_records_exactly(engine, dims, fmt):
     """The 16-bit records (packed 3-byte records in 3 x 3 x 4 bricks) on sides
-    that are not multiples of the bricks: with zero diffusivity every spin stays
+    that are not multiples of the bricks, or whose brick counts along y and z
+    are even and laid out padded to odd ones: with zero diffusivity every spin stays
     at the centre of its voxel, so it must see that voxel's label, and its
     magnetisation must be the oracle's for the field the record holds (the
     float32 field quantized as the engine quantizes it): labels equal, M within
