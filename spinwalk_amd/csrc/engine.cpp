@@ -415,8 +415,7 @@ static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
         if (!want) continue;
         size_t bytes;
         if (f == SW_FIELD_F16 || f == SW_FIELD_BF16 || f == SW_FIELD_Q16) {   // packed 3-byte records
-            bytes = (size_t)(sw::pack_brick_count(nx, sw::kPack16.bx) * sw::pack_pitch(ny, sw::kPack16.by) *
-                             sw::pack_pitch(nz, sw::kPack16.bz)) * 128;
+            bytes = (size_t)sw::pack_bytes(nx, ny, nz, sw::kPack16, sw::pack_super(nx, ny, nz, sw::kPack16));
         } else {
             const sw::BrickShape sh = f == SW_FIELD_F32 ? sw::kBrick8 : sw::kBrick4;
             bytes = ((size_t)(sw::brick_count(nx, sh.lbx) * sw::brick_count(ny, sh.lby) * sw::brick_count(nz, sh.lbz))
@@ -428,9 +427,7 @@ static int phantom_finish(sw_engine *e, sw_phantom *ph, uint32_t formats) {
         SW_HIP(e, sw::launch_records(ph->mask, ph->field, nx, ny, nz, kind,
                                      1.0f / (f == SW_FIELD_Q16 ? ph->q_scale : ph->half_scale), ph->rec[f], s));
         if (f == SW_FIELD_F32) {         // the production walks' copies: packed for batches, SoA for single walks
-            const size_t bytes = (size_t)(sw::pack_brick_count(nx, sw::kPackF32.bx) *
-                                          sw::pack_pitch(ny, sw::kPackF32.by) *
-                                          sw::pack_pitch(nz, sw::kPackF32.bz)) * 128;
+            const size_t bytes = (size_t)sw::pack_bytes(nx, ny, nz, sw::kPackF32, sw::pack_super(nx, ny, nz, sw::kPackF32));
             for (int soa = 0; soa < 2; soa++) {
                 void *&r = soa ? ph->rec_soa : ph->rec_packed;
                 SW_HIP(e, hipMalloc(&r, bytes));
@@ -800,16 +797,23 @@ int fill_walk_args(sw_engine *e, const sw_phantom *ph, const sw_sequence *q, con
 
     std::memset(&a, 0, sizeof a);
     a.rec = ph->rec[field];
-    if (field == SW_FIELD_F32 && o->rng == SW_RNG_PHILOX) {
-        // a single production walk gathers the float32 field from the
-        // structure-of-arrays bricks (a batch swaps in the packed records)
-        if (!ph->rec_soa || !ph->rec_packed) return fail(e, SW_E_INVALID, "the phantom has no production float32 records");
-        a.rec = ph->rec_soa;
-        a.nb_y = (uint32_t)sw::pack_pitch(ph->dims[1], sw::kPackF32.by);
-        a.nb_z = (uint32_t)sw::pack_pitch(ph->dims[2], sw::kPackF32.bz);
-    } else if (field == SW_FIELD_F16 || field == SW_FIELD_BF16 || field == SW_FIELD_Q16) {
-        a.nb_y = (uint32_t)sw::pack_pitch(ph->dims[1], sw::kPack16.by);
-        a.nb_z = (uint32_t)sw::pack_pitch(ph->dims[2], sw::kPack16.bz);
+    const bool packed = field == SW_FIELD_F16 || field == SW_FIELD_BF16 || field == SW_FIELD_Q16 ||
+                        (field == SW_FIELD_F32 && o->rng == SW_RNG_PHILOX);
+    if (packed) {
+        if (field == SW_FIELD_F32) {
+            // a single production walk gathers the float32 field from the
+            // structure-of-arrays bricks (a batch swaps in the packed records)
+            if (!ph->rec_soa || !ph->rec_packed) return fail(e, SW_E_INVALID, "the phantom has no production float32 records");
+            a.rec = ph->rec_soa;
+        }
+        const sw::PackShape &S = field == SW_FIELD_F32 ? sw::kPackF32 : sw::kPack16;
+        const bool sup = sw::pack_super(ph->dims[0], ph->dims[1], ph->dims[2], S);
+        a.super_blocks = sup ? 1u : 0u;
+        a.nb_y = (uint32_t)sw::pack_pitch_y(ph->dims[1], S, sup);
+        a.nb_z = (uint32_t)sw::pack_pitch_z(ph->dims[2], S, sup);
+        // the walk forms the records' offsets with 32-bit strides
+        if ((uint64_t)a.nb_y * a.nb_z * (sup ? sw::kSuperLines : 1) * 128 >= (1ull << 32))
+            return fail(e, SW_E_INVALID, "the phantom's y-z planes are too large for the packed records (4 GiB a slab)");
     } else {
         const sw::BrickShape sh = field == SW_FIELD_F32 ? sw::kBrick8 : sw::kBrick4;
         a.nb_y = (uint32_t)sw::brick_count(ph->dims[1], sh.lby);
@@ -997,6 +1001,7 @@ struct sw_batch {
     std::vector<Reduce> reduces;
     int rng = 0, field = 0, prec = 0;
     bool idx32 = false;
+    bool mixed = false;                          // 64-bit mode over items in both packed layouts
     size_t lds = 0;                              // LDS of a workgroup: per-axis tables, then tissue tables (32-bit mode)
     // what the chunk table is built from: per walk unit (a single tile, or a
     // fused group of tiles), its CROSS_FOV group, spins, steps per spin, phantom
@@ -1046,7 +1051,8 @@ hipError_t batch_enqueue(const sw_batch *b, hipStream_t s) {
     for (const auto &g : b->groups) {
         const void *items = g.views == 1 ? (const void *)b->d_items : (const void *)b->d_fused;
         err = sw::launch_walk_batch(items, b->d_chunks + g.offset, g.n_slots, b->sched.chunk_spins,
-                                    b->sched.spins_per_lane, g.views, b->rng, b->field, b->prec, g.cross, b->idx32,
+                                    b->sched.spins_per_lane, g.views, b->rng, b->field, b->prec, g.cross,
+                                    b->idx32 ? sw::ADDR_32 : b->mixed ? sw::ADDR_64_MIXED : sw::ADDR_64,
                                     lds, s);
         if (err != hipSuccess) return err;
     }
@@ -1323,6 +1329,7 @@ SW_API int sw_batch_create(sw_engine *e, const sw_batch_item *items, uint32_t n_
         }
     }
     if (!b->idx32) b->lds = 0;
+    for (uint32_t i = 0; i < n_items && !b->idx32; i++) b->mixed = b->mixed || args[i].super_blocks;
 
     // ---- device tables
     SW_HIP(e, hipMalloc((void **)&b->d_counters, 4 * n_items * sizeof(unsigned long long)));

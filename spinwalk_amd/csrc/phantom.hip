@@ -75,11 +75,12 @@ __global__ void records_kernel(const uint8_t *mask, const float *field, uint64_t
 // Row-major mask/field -> packed 3-byte 16-bit records (kPack16).
 template <int KIND>
 __global__ void packed16_records_kernel(const uint8_t *mask, const float *field, uint64_t n, uint32_t ny,
-                                        uint32_t nz, uint64_t nb_y, uint64_t nb_z, float inv_scale, uint8_t *out) {
+                                        uint32_t nz, uint64_t nb_y, uint64_t nb_z, bool sup, float inv_scale,
+                                        uint8_t *out) {
     constexpr PackShape S = kPack16;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t iz = v % nz, iy = (v / nz) % ny, ix = v / ((uint64_t)nz * ny);
-        const uint64_t b = ((ix / S.bx) * nb_y + iy / S.by) * nb_z + iz / S.bz;
+        const uint64_t b = pack_line(ix / S.bx, iy / S.by, iz / S.bz, nb_y, nb_z, sup);
         const uint32_t j = (uint32_t)(((ix % S.bx) * S.by + iy % S.by) * S.bz + iz % S.bz);
         uint8_t *r = out + b * 128 + S.rec_bytes * j;
         const uint32_t h = field16<KIND>(field, v, inv_scale);
@@ -93,11 +94,11 @@ __global__ void packed16_records_kernel(const uint8_t *mask, const float *field,
 // records (kPackF32) or structure-of-arrays records (kSoaF32).
 template <bool SOA>
 __global__ void f32_records_kernel(const uint8_t *mask, const float *field, uint64_t n, uint32_t ny, uint32_t nz,
-                                   uint64_t nb_y, uint64_t nb_z, uint8_t *out) {
+                                   uint64_t nb_y, uint64_t nb_z, bool sup, uint8_t *out) {
     constexpr PackShape S = SOA ? kSoaF32 : kPackF32;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t iz = v % nz, iy = (v / nz) % ny, ix = v / ((uint64_t)nz * ny);
-        const uint64_t b = ((ix / S.bx) * nb_y + iy / S.by) * nb_z + iz / S.bz;
+        const uint64_t b = pack_line(ix / S.bx, iy / S.by, iz / S.bz, nb_y, nb_z, sup);
         const uint32_t j = (uint32_t)(((ix % S.bx) * S.by + iy % S.by) * S.bz + iz % S.bz);
         const uint32_t f = __float_as_uint(field ? field[v] : 0.f);
         if constexpr (SOA) {
@@ -120,13 +121,14 @@ hipError_t launch_f32_records(const uint8_t *mask, const float *field, uint64_t 
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 16384) blocks = 16384;
-    const uint64_t nby = pack_pitch(ny, kPackF32.by), nbz = pack_pitch(nz, kPackF32.bz);
+    const bool sup = pack_super(nx, ny, nz, kPackF32);
+    const uint64_t nby = pack_pitch_y(ny, kPackF32, sup), nbz = pack_pitch_z(nz, kPackF32, sup);
     if (soa)
         hipLaunchKernelGGL(f32_records_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, mask, field, n,
-                           (uint32_t)ny, (uint32_t)nz, nby, nbz, (uint8_t *)out);
+                           (uint32_t)ny, (uint32_t)nz, nby, nbz, sup, (uint8_t *)out);
     else
         hipLaunchKernelGGL(f32_records_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, mask, field, n,
-                           (uint32_t)ny, (uint32_t)nz, nby, nbz, (uint8_t *)out);
+                           (uint32_t)ny, (uint32_t)nz, nby, nbz, sup, (uint8_t *)out);
     return hipGetLastError();
 }
 
@@ -155,17 +157,18 @@ hipError_t launch_records(const uint8_t *mask, const float *field, uint64_t nx, 
         hipLaunchKernelGGL((records_kernel<kBrick4.lbx, kBrick4.lby, kBrick4.lbz, 3>), g, b, 0, s, mask, field, n,
                            (uint32_t)ny, (uint32_t)nz, nby, nbz, inv_scale, out);
     } else {
-        const uint64_t nby = pack_pitch(ny, kPack16.by), nbz = pack_pitch(nz, kPack16.bz);
+        const bool sup = pack_super(nx, ny, nz, kPack16);
+        const uint64_t nby = pack_pitch_y(ny, kPack16, sup), nbz = pack_pitch_z(nz, kPack16, sup);
         uint8_t *o = (uint8_t *)out;
         if (kind == 1)
             hipLaunchKernelGGL((packed16_records_kernel<1>), g, b, 0, s, mask, field, n, (uint32_t)ny, (uint32_t)nz,
-                               nby, nbz, inv_scale, o);
+                               nby, nbz, sup, inv_scale, o);
         else if (kind == 2)
             hipLaunchKernelGGL((packed16_records_kernel<2>), g, b, 0, s, mask, field, n, (uint32_t)ny, (uint32_t)nz,
-                               nby, nbz, inv_scale, o);
+                               nby, nbz, sup, inv_scale, o);
         else
             hipLaunchKernelGGL((packed16_records_kernel<4>), g, b, 0, s, mask, field, n, (uint32_t)ny, (uint32_t)nz,
-                               nby, nbz, inv_scale, o);
+                               nby, nbz, sup, inv_scale, o);
     }
     return hipGetLastError();
 }

@@ -22,14 +22,14 @@ WalkMode walk_mode(const WalkArgs &a, int field, int rng) {
 }
 
 hipError_t launch_walk_batch(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                             uint32_t spl, int views, int rng, int field, int prec, bool cross, bool idx32,
+                             uint32_t spl, int views, int rng, int field, int prec, bool cross, int addr,
                              size_t lds, hipStream_t s) {
     if (n_slots == 0) return hipSuccess;
     if (n_slots % 8 || threads == 0 || threads > 256 || threads % 64 || spl == 0) return hipErrorInvalidValue;
     if (rng == 1)
-        return launch_walk_batch_replay(items, chunks, n_slots, threads, spl, views, field, prec, cross, idx32, lds, s);
+        return launch_walk_batch_replay(items, chunks, n_slots, threads, spl, views, field, prec, cross, addr, lds, s);
     if (rng == 0)
-        return launch_walk_batch_philox(items, chunks, n_slots, threads, spl, views, field, prec, cross, idx32, lds, s);
+        return launch_walk_batch_philox(items, chunks, n_slots, threads, spl, views, field, prec, cross, addr, lds, s);
     return hipErrorInvalidValue;
 }
 

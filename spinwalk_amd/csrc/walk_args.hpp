@@ -78,11 +78,44 @@ static_assert(kSoaF32.bx == kPackF32.bx && kSoaF32.by == kPackF32.by && kSoaF32.
 static_assert(kPack16.nvox == kPack16.bx * kPack16.by * kPack16.bz &&
               (kPack16.nvox - 1) * kPack16.rec_bytes + 4 <= 128, "16-bit records leave their line");
 SW_HD_INLINE uint64_t pack_brick_count(uint64_t n, int b) { return (n + b - 1) / b; }
-// Bricks along y and z as laid out: the count rounded up to an odd number, so
-// that bricks adjacent in y (nb_z lines apart) or in x (nb_y nb_z lines apart)
-// are an odd number of lines apart, not a multiple of a large power of two
-// (the bricks of the padding slots are never gathered).
-SW_HD_INLINE uint64_t pack_pitch(uint64_t n, int b) { return pack_brick_count(n, b) | 1; }
+// Packed arrays that the walk addresses in its 32-bit mode (below 4 GiB, with
+// the per-axis tables in LDS) group their bricks in super-blocks of 4 x 4 x 2
+// bricks, each a contiguous run of 32 lines (bricks row-major inside), so that
+// the lines of neighbouring bricks sit together in memory; super-blocks are
+// row-major.  Larger arrays keep single bricks (super-blocks of 1 x 1 x 1).
+// Along y and z the count of super-blocks (nb_y, nb_z of WalkArgs) is rounded
+// up to an odd number, so that super-blocks adjacent in y or x are an odd
+// multiple of the super-block's size apart, not a multiple of a large power of
+// two (the padding is never gathered).
+struct SuperShape { int sx, sy, sz; };
+constexpr SuperShape kSuper = {4, 4, 2};
+constexpr int kSuperLines = kSuper.sx * kSuper.sy * kSuper.sz;
+SW_HD_INLINE uint64_t pack_super_count(uint64_t n, int b, int s) { return (pack_brick_count(n, b) + s - 1) / s; }
+SW_HD_INLINE uint64_t pack_pitch_y(uint64_t ny, const PackShape &S, bool sup) {
+    return pack_super_count(ny, S.by, sup ? kSuper.sy : 1) | 1;
+}
+SW_HD_INLINE uint64_t pack_pitch_z(uint64_t nz, const PackShape &S, bool sup) {
+    return pack_super_count(nz, S.bz, sup ? kSuper.sz : 1) | 1;
+}
+SW_HD_INLINE uint64_t pack_bytes(uint64_t nx, uint64_t ny, uint64_t nz, const PackShape &S, bool sup) {
+    return pack_super_count(nx, S.bx, sup ? kSuper.sx : 1) * pack_pitch_y(ny, S, sup) * pack_pitch_z(nz, S, sup) *
+           (sup ? kSuperLines : 1) * 128;
+}
+// Whether the packed array of an nx x ny x nz phantom is laid out in super-blocks.
+SW_HD_INLINE bool pack_super(uint64_t nx, uint64_t ny, uint64_t nz, const PackShape &S) {
+    return pack_bytes(nx, ny, nz, S, true) < (1ull << 32) && (nx + ny + nz) * 4 <= 65536;
+}
+// How a launch addresses the records: byte offsets from the per-axis LDS tables
+// (32-bit mode), or computed per lookup in 64-bit arithmetic for arrays in single
+// bricks, or for a batch whose items mix both layouts (the per-item flag
+// super_blocks is read at each lookup).
+enum AddrMode : int { ADDR_64 = 0, ADDR_32 = 1, ADDR_64_MIXED = 2 };
+// Line of brick (bx, by, bz) in a packed array of pitch nb_y, nb_z.
+SW_HD_INLINE uint64_t pack_line(uint64_t bx, uint64_t by, uint64_t bz, uint64_t nb_y, uint64_t nb_z, bool sup) {
+    if (!sup) return (bx * nb_y + by) * nb_z + bz;
+    const uint64_t sb = ((bx / kSuper.sx) * nb_y + by / kSuper.sy) * nb_z + bz / kSuper.sz;
+    return sb * kSuperLines + ((bx % kSuper.sx) * kSuper.sy + by % kSuper.sy) * kSuper.sz + bz % kSuper.sz;
+}
 
 SW_HD_INLINE uint64_t brick_count(uint64_t n, int lb) { return (n + (1ull << lb) - 1) >> lb; }
 
@@ -133,6 +166,7 @@ struct WalkArgs {
     uint32_t        n_sub;
     uint32_t        tissue_lds;   // 1: each workgroup copies the tissue tables to LDS (tissue_lds_words)
     uint32_t        pairs;        // 1: a single production walk takes two attempts per trip of its step loop
+    uint32_t        super_blocks; // 1: the packed records are grouped in super-blocks (kSuper)
     const RfPulse  *rf;           // [n_rf]; entry 0 unused (excitation)
     uint32_t        n_rf;
     const int32_t  *TE_t;         uint32_t n_te;

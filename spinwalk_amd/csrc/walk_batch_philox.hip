@@ -5,9 +5,9 @@
 namespace sw {
 
 hipError_t launch_walk_batch_philox(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                                    uint32_t spl, int views, int field, int prec, bool cross, bool idx32, size_t lds,
+                                    uint32_t spl, int views, int field, int prec, bool cross, int addr, size_t lds,
                                     hipStream_t s) {
-    return launch_batch_rng<PhiloxRng>(items, chunks, n_slots, threads, spl, views, field, prec, cross, idx32, lds, s);
+    return launch_batch_rng<PhiloxRng>(items, chunks, n_slots, threads, spl, views, field, prec, cross, addr, lds, s);
 }
 
 }  // namespace sw

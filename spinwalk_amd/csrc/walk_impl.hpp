@@ -163,6 +163,7 @@ __device__ __forceinline__ T load_rec(const void *base, uint64_t i) {
 struct RecF32 {                          // {float field_T, uint32 label}
     static constexpr BrickShape shape = kBrick8;
     static constexpr bool kPacked = false;
+    static constexpr bool kMixedLayout = false;
     using T = uint2;
     __device__ static uint32_t label(T r) { return r.y; }
     template <class R> __device__ static R field(const WalkArgs &a, T r) {
@@ -173,6 +174,7 @@ struct RecF32 {                          // {float field_T, uint32 label}
 struct RecNone {                         // label only: {0, uint8 label, pad}
     static constexpr BrickShape shape = kBrick4;
     static constexpr bool kPacked = false;
+    static constexpr bool kMixedLayout = false;
     using T = uint32_t;
     __device__ static uint32_t label(T r) { return r >> 16; }
     template <class R> __device__ static R field(const WalkArgs &, T) { return (R)0; }
@@ -184,6 +186,7 @@ struct RecNone {                         // label only: {0, uint8 label, pad}
 typedef uint32_t __attribute__((aligned(1))) u32_unaligned;
 struct Rec16 {
     static constexpr bool kPacked = true;
+    static constexpr bool kMixedLayout = false;
     static constexpr PackShape pack = kPack16;
     using T = uint32_t;                  // the 4 bytes loaded
     __device__ static uint32_t label(T r) { return r & 0xFFu; }
@@ -215,6 +218,7 @@ struct RecBF16 : Rec16 {                 // bf16 field_T / half_scale
 typedef uint64_t __attribute__((aligned(1))) u64_unaligned;
 struct RecF32P {
     static constexpr bool kPacked = true;
+    static constexpr bool kMixedLayout = false;
     static constexpr PackShape pack = kPackF32;
     using T = uint2;                     // the 8 bytes loaded: x = bytes 0-3, y = bytes 4-7
     __device__ static uint32_t label(T r) { return r.x & 0xFFu; }
@@ -235,6 +239,7 @@ struct RecF32P {
 // offset mod 128.  Two aligned loads from one line.
 struct RecF32S {
     static constexpr bool kPacked = true;
+    static constexpr bool kMixedLayout = false;
     static constexpr PackShape pack = kSoaF32;
     using T = uint2;                     // {field bits, label}, as RecF32
     __device__ static uint32_t label(T r) { return r.y; }
@@ -247,6 +252,10 @@ struct RecF32S {
         return r;
     }
 };
+
+// A packed record type whose arrays, in one batch, come in both layouts: the
+// 64-bit lookups read each item's super_blocks flag (ADDR_64_MIXED).
+template <class Rec> struct MixedLayout : Rec { static constexpr bool kMixedLayout = true; };
 
 template <class Rec, bool IDX32, class I>
 __device__ __forceinline__ typename Rec::T rec_load(const void *base, I off) {
@@ -353,15 +362,40 @@ template <bool IDX32> struct Vox {
 extern __shared__ uint32_t axis_tab[];
 
 // For the packed bricks the index is the record's byte offset, separable in the
-// same way (w = 5 bytes a record):
-// X = (ix / bx) * nb_y * nb_z * 128 + (ix mod bx) * by * bz * w,
-// Y = (iy / by) * nb_z * 128 + (iy mod by) * bz * w, Z = (iz / bz) * 128 + (iz mod bz) * w.
+// same way (w bytes a record, brick coordinates b = i / brick side, super-blocks
+// of sx x sy x sz bricks and SL = sx sy sz lines, pack_line):
+// X = ((bx / sx) * nb_y * nb_z * SL + (bx mod sx) * sy * sz) * 128 + (ix mod bx) * by * bz * w,
+// Y = ((by / sy) * nb_z * SL + (by mod sy) * sz) * 128 + (iy mod by) * bz * w,
+// Z = ((bz / sz) * SL + bz mod sz) * 128 + (iz mod bz) * w.
+// Subscripts, the in-super-block parts and the strides of a super-block along
+// x (nb_y nb_z SL 128 bytes, checked below 2^32 on the host) and y fit 32 bits:
+// the divisions run in 32-bit arithmetic and each part takes one 32 x 32 -> 64
+// multiply-add in the 64-bit mode.
+template <class Rec, int SX, int SY, int SZ, class I>
+__device__ __forceinline__ I pack_part_of(const WalkArgs &a, int axis, I i_) {
+    constexpr PackShape S = Rec::pack;
+    constexpr uint32_t SL = SX * SY * SZ;
+    const uint32_t i = (uint32_t)i_;
+    if (axis == 0) {
+        const uint32_t b = i / S.bx;
+        const uint32_t stride = a.nb_y * a.nb_z * (SL * 128);
+        return (I)(b / SX) * (I)stride +
+               (I)((b % SX) * (uint32_t)(SY * SZ * 128) + (i % S.bx) * (uint32_t)(S.by * S.bz * S.rec_bytes));
+    }
+    if (axis == 1) {
+        const uint32_t b = i / S.by;
+        const uint32_t stride = a.nb_z * (SL * 128);
+        return (I)(b / SY) * (I)stride + (I)((b % SY) * (uint32_t)(SZ * 128) + (i % S.by) * (uint32_t)(S.bz * S.rec_bytes));
+    }
+    const uint32_t b = i / S.bz;
+    return (I)(b / SZ) * (I)(SL * 128) + (I)((b % SZ) * 128u + (i % S.bz) * (uint32_t)S.rec_bytes);
+}
+
+// The array's layout is uniform across the launch: a scalar branch picks the form.
 template <class Rec, class I>
 __device__ __forceinline__ I pack_part(const WalkArgs &a, int axis, I i) {
-    constexpr PackShape S = Rec::pack;
-    if (axis == 0) return (i / S.bx) * (I)a.nb_y * (I)a.nb_z * 128 + (i % S.bx) * (S.by * S.bz * S.rec_bytes);
-    if (axis == 1) return (i / S.by) * (I)a.nb_z * 128 + (i % S.by) * (S.bz * S.rec_bytes);
-    return (i / S.bz) * 128 + (i % S.bz) * S.rec_bytes;
+    if (a.super_blocks) return pack_part_of<Rec, kSuper.sx, kSuper.sy, kSuper.sz>(a, axis, i);
+    return pack_part_of<Rec, 1, 1, 1>(a, axis, i);
 }
 
 template <class Rec>
@@ -391,7 +425,16 @@ __device__ __forceinline__ uint64_t record_of(const WalkArgs &a, uint64_t ix, ui
         return axis_tab[(uint32_t)ix] + axis_tab[nx + (uint32_t)iy] + axis_tab[nxy + (uint32_t)iz];
     }
     if constexpr (Rec::kPacked) {
-        return pack_part<Rec, uint64_t>(a, 0, ix) + pack_part<Rec, uint64_t>(a, 1, iy) + pack_part<Rec, uint64_t>(a, 2, iz);
+        // The 64-bit mode walks arrays in single bricks (an array in super-blocks
+        // always fits the 32-bit mode), except in a batch that mixes layouts.
+        if constexpr (Rec::kMixedLayout) {
+            if (a.super_blocks)
+                return pack_part_of<Rec, kSuper.sx, kSuper.sy, kSuper.sz>(a, 0, ix) +
+                       pack_part_of<Rec, kSuper.sx, kSuper.sy, kSuper.sz>(a, 1, iy) +
+                       pack_part_of<Rec, kSuper.sx, kSuper.sy, kSuper.sz>(a, 2, iz);
+        }
+        return pack_part_of<Rec, 1, 1, 1>(a, 0, ix) + pack_part_of<Rec, 1, 1, 1>(a, 1, iy) +
+               pack_part_of<Rec, 1, 1, 1>(a, 2, iz);
     } else {
         constexpr int LX = Rec::shape.lbx, LY = Rec::shape.lby, LZ = Rec::shape.lbz;
         const uint64_t inner = ((ix & ((1u << LX) - 1)) << (LY + LZ)) | ((iy & ((1u << LY) - 1)) << LZ) |
@@ -423,7 +466,7 @@ __device__ __forceinline__ Vox<IDX32> locate_outside(const WalkArgs &a, double v
     if (ind >= 0 && ind < a.matrix_length) {                    // kernels.cu:141
         const uint64_t u = (uint64_t)ind, nz = (uint64_t)a.nz, ny = (uint64_t)a.ny;
         v.ok = true;
-        v.rec = (RecIdx<IDX32>)record_of<Rec, false>(a, u / (nz * ny), (u / nz) % ny, u % nz);
+        v.rec = (RecIdx<IDX32>)record_of<Rec, IDX32>(a, u / (nz * ny), (u / nz) % ny, u % nz);   // each in range
     }
     return v;
 }
@@ -1502,7 +1545,8 @@ template <class F>
 WalkMode mode_of(const WalkArgs &a) {
     uint64_t bytes;
     if constexpr (F::kPacked) {
-        bytes = pack_brick_count(a.nx, F::pack.bx) * a.nb_y * a.nb_z * 128;
+        bytes = a.super_blocks ? pack_super_count(a.nx, F::pack.bx, kSuper.sx) * a.nb_y * a.nb_z * kSuperLines * 128
+                               : pack_brick_count(a.nx, F::pack.bx) * a.nb_y * a.nb_z * 128;
     } else {
         constexpr int lb = F::shape.lbx + F::shape.lby + F::shape.lbz;
         bytes = (brick_count(a.nx, F::shape.lbx) * a.nb_y * a.nb_z << lb) * sizeof(typename F::T);
@@ -1520,6 +1564,9 @@ hipError_t launch_fp_loop(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 bloc
     const WalkMode m = mode_of<F>(a);
     const bool idx32 = m.idx32;
     const size_t lds = m.lds;
+    if constexpr (F::kPacked) {
+        if (!idx32 && a.super_blocks) return hipErrorInvalidValue;   // super-blocks always fit the 32-bit mode
+    }
     if (a.cross_fov) {
         if (idx32) hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, true, PAIRS>), grid, block, lds, s, a);
         else       hipLaunchKernelGGL((walk_kernel<Rng, F, P, true, false, PAIRS>), grid, block, 0, s, a);
@@ -1540,8 +1587,16 @@ hipError_t launch_fp(const WalkArgs &a, hipStream_t s, dim3 grid, dim3 block) {
 
 template <class Rng, class F, class P, int NV>
 hipError_t launch_batch_nv(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                           uint32_t spl, bool cross, bool idx32, size_t lds, hipStream_t s) {
+                           uint32_t spl, bool cross, int addr, size_t lds, hipStream_t s) {
     const dim3 grid(n_slots), block(threads);
+    if constexpr (F::kPacked) {
+        if (addr == ADDR_64_MIXED) {
+            if (cross) hipLaunchKernelGGL((walk_batch_kernel<Rng, MixedLayout<F>, P, true, false, NV>), grid, block, lds, s, items, chunks, spl);
+            else       hipLaunchKernelGGL((walk_batch_kernel<Rng, MixedLayout<F>, P, false, false, NV>), grid, block, lds, s, items, chunks, spl);
+            return hipGetLastError();
+        }
+    }
+    const bool idx32 = addr == ADDR_32;
     if (cross) {
         if (idx32) hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, true, NV>), grid, block, lds, s, items, chunks, spl);
         else       hipLaunchKernelGGL((walk_batch_kernel<Rng, F, P, true, false, NV>), grid, block, lds, s, items, chunks, spl);
@@ -1555,11 +1610,11 @@ hipError_t launch_batch_nv(const void *items, const BatchChunk *chunks, uint32_t
 // views: 1, or kMaxViews for fused items (FP32 and Kahan phases only).
 template <class Rng, class F, class P>
 hipError_t launch_batch_fp(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                           uint32_t spl, int views, bool cross, bool idx32, size_t lds, hipStream_t s) {
-    if (views == 1) return launch_batch_nv<Rng, F, P, 1>(items, chunks, n_slots, threads, spl, cross, idx32, lds, s);
+                           uint32_t spl, int views, bool cross, int addr, size_t lds, hipStream_t s) {
+    if (views == 1) return launch_batch_nv<Rng, F, P, 1>(items, chunks, n_slots, threads, spl, cross, addr, lds, s);
     if constexpr (sizeof(typename P::R) == 4) {
         if (views == kMaxViews)
-            return launch_batch_nv<Rng, F, P, kMaxViews>(items, chunks, n_slots, threads, spl, cross, idx32, lds, s);
+            return launch_batch_nv<Rng, F, P, kMaxViews>(items, chunks, n_slots, threads, spl, cross, addr, lds, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1575,23 +1630,23 @@ template <class Rng> using RecF32Single = typename std::conditional<Rng::kReplay
 
 template <class Rng>
 hipError_t launch_batch_rng(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                            uint32_t spl, int views, int field, int prec, bool cross, bool idx32, size_t lds,
+                            uint32_t spl, int views, int field, int prec, bool cross, int addr, size_t lds,
                             hipStream_t s) {
     if (prec == 2) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF64>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32Batch<Rng>, PhaseF64>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF64>(items, chunks, n_slots, threads, spl, views, cross, addr, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32Batch<Rng>, PhaseF64>(items, chunks, n_slots, threads, spl, views, cross, addr, lds, s);
     } else if (prec == 1) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32Batch<Rng>, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
-        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
-        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
-        if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, addr, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32Batch<Rng>, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, addr, lds, s);
+        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, addr, lds, s);
+        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, addr, lds, s);
+        if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseKahan>(items, chunks, n_slots, threads, spl, views, cross, addr, lds, s);
     } else if (prec == 0) {
-        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
-        if (field == 1) return launch_batch_fp<Rng, RecF32Batch<Rng>, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
-        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
-        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
-        if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, idx32, lds, s);
+        if (field == 0) return launch_batch_fp<Rng, RecNone, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, addr, lds, s);
+        if (field == 1) return launch_batch_fp<Rng, RecF32Batch<Rng>, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, addr, lds, s);
+        if (field == 2) return launch_batch_fp<Rng, RecF16, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, addr, lds, s);
+        if (field == 3) return launch_batch_fp<Rng, RecBF16, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, addr, lds, s);
+        if (field == 4) return launch_batch_fp<Rng, RecQ16, PhaseF32>(items, chunks, n_slots, threads, spl, views, cross, addr, lds, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1626,10 +1681,10 @@ hipError_t launch_rng(const WalkArgs &a, int field, int prec, hipStream_t s) {
 hipError_t launch_walk_replay(const WalkArgs &a, int field, int prec, hipStream_t s);
 hipError_t launch_walk_philox(const WalkArgs &a, int field, int prec, hipStream_t s);
 hipError_t launch_walk_batch_replay(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                                    uint32_t spl, int views, int field, int prec, bool cross, bool idx32, size_t lds,
+                                    uint32_t spl, int views, int field, int prec, bool cross, int addr, size_t lds,
                                     hipStream_t s);
 hipError_t launch_walk_batch_philox(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                                    uint32_t spl, int views, int field, int prec, bool cross, bool idx32, size_t lds,
+                                    uint32_t spl, int views, int field, int prec, bool cross, int addr, size_t lds,
                                     hipStream_t s);
 
 }  // namespace sw

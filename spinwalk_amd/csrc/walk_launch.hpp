@@ -22,9 +22,10 @@ WalkMode walk_mode(const WalkArgs &a, int field, int rng);
 // spins (each lane walks spl spins, one after the other):
 // items[] (WalkArgs, or FusedArgs when views == kMaxViews; prec 0 or 1 only)
 // and chunks[] in device memory; every item shares rng, field, prec, cross and
-// the index mode (idx32 with `lds` bytes covering every item's tables).
+// the address mode (AddrMode; ADDR_32 with `lds` bytes covering every item's
+// tables).
 hipError_t launch_walk_batch(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                             uint32_t spl, int views, int rng, int field, int prec, bool cross, bool idx32,
+                             uint32_t spl, int views, int rng, int field, int prec, bool cross, int addr,
                              size_t lds, hipStream_t s);
 
 // Production draws of spins [spin0, spin0+n_spins), attempts [0, n_att).

@@ -10,9 +10,9 @@ hipError_t launch_walk_replay(const WalkArgs &a, int field, int prec, hipStream_
 }
 
 hipError_t launch_walk_batch_replay(const void *items, const BatchChunk *chunks, uint32_t n_slots, uint32_t threads,
-                                    uint32_t spl, int views, int field, int prec, bool cross, bool idx32, size_t lds,
+                                    uint32_t spl, int views, int field, int prec, bool cross, int addr, size_t lds,
                                     hipStream_t s) {
-    return launch_batch_rng<ReplayRng>(items, chunks, n_slots, threads, spl, views, field, prec, cross, idx32, lds, s);
+    return launch_batch_rng<ReplayRng>(items, chunks, n_slots, threads, spl, views, field, prec, cross, addr, lds, s);
 }
 
 }  // namespace sw

@@ -242,6 +242,31 @@ def test_batch_fuses_same_trajectory_tiles(engine):
         b.close()
 
 
+def test_batch_mixes_record_layouts(engine):
+    """The records of a phantom whose per-axis tables overflow LDS (16400 x 4 x 4
+    voxels) stay in single bricks and are addressed in the 64-bit mode; a small
+    phantom's records are grouped in super-blocks of bricks and addressed from
+    the LDS tables. In one batch both tiles run in the 64-bit mode, which then
+    reads each tile's layout: every tile equals its separate walk bit for bit,
+    in both precisions the fused and single batches use."""
+    import torch
+    m_big, f_big = cases.blocky_phantom((16400, 4, 4), n_labels=2, seed=51, block=3)
+    m_small, f_small = cases.cylinder_phantom(n=24, fov_um=48.0, radius_um=5.0, n_cyl=3, seed=52)
+    gre = cases.gre(2, TE=2000, dt=50, P_XY=[[1, 0.5], [0.5, 1]])
+    for prec in (E.SW_PREC_F32, E.SW_PREC_F32_KAHAN):
+        tiles = [_tile(engine, torch, cases.make_case(gre, m_small, f_small * 30, 48.0, n_spins=1500),
+                       precision=prec),
+                 _tile(engine, torch, cases.make_case(gre, m_big, f_big * 30, (16400.0, 4.0, 4.0), n_spins=1500),
+                       precision=prec)]
+        refs = [_separate(engine, torch, t) for t in tiles]
+        b = engine.batch(tiles)
+        st = b.run()
+        torch.cuda.synchronize()
+        for t, r, s in zip(tiles, refs, st["items"]):
+            _check(t, r, s)
+        b.close()
+
+
 def test_empty_inputs(engine):
     """Zero spins everywhere an entry point takes spins: a walk of none is a
     no-op that reports zero steps, echo sums over no spins are zero (and are
