@@ -324,9 +324,10 @@ SW_API int  sw_batch_set_schedule(sw_batch *b, const sw_batch_schedule *s);
 /* Times full runs of the batch on `stream` under candidate schedules (chunk size,
  * the band counts of the class kinds present, spins per lane, resident waves
  * per SIMD, visiting-order brick), keeps the fastest and recaptures
- * the graph.  At most max_trials timed runs after two of the current
- * schedule (0: no limit).  *best_ms / *first_ms: the kept schedule's run and the
- * starting schedule's. */
+ * the graph.  Each trial times a schedule as the faster of two full runs; at
+ * most max_trials trials after the current schedule's (0: no limit), which
+ * follows one untimed run.  *best_ms / *first_ms: the kept schedule's time and
+ * the starting schedule's. */
 SW_API int  sw_batch_tune(sw_batch *b, void *stream, uint32_t max_trials, sw_batch_schedule *best,
                           float *best_ms, float *first_ms);
 
