@@ -323,7 +323,8 @@ SW_API int  sw_batch_get_schedule(const sw_batch *b, sw_batch_schedule *out);
 SW_API int  sw_batch_set_schedule(sw_batch *b, const sw_batch_schedule *s);
 /* Times full runs of the batch on `stream` under candidate schedules (chunk size,
  * the band counts of the class kinds present, spins per lane, resident waves
- * per SIMD, visiting-order brick), keeps the fastest and recaptures
+ * per SIMD, visiting-order brick, then every pair of chunk size and resident
+ * waves, since the two interact), keeps the fastest and recaptures
  * the graph.  Each trial times a schedule as the faster of two full runs; at
  * most max_trials trials after the current schedule's (0: no limit), which
  * follows one untimed run.  *best_ms / *first_ms: the kept schedule's time and

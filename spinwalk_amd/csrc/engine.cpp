@@ -1490,9 +1490,9 @@ SW_API int sw_batch_set_schedule(sw_batch *b, const sw_batch_schedule *sc) {
 
 // Coordinate search over the schedule: starting from the current one, each
 // parameter in turn (chunk size, the band counts of the class kinds the batch
-// has, spins per lane, resident waves per SIMD, visiting-order brick, then the
-// chunk size and the waves once more) takes the candidate value that times
-// fastest, a candidate having to beat the best by 1 %.  Each trial is the faster
+// has, spins per lane, resident waves per SIMD, visiting-order brick, then every
+// pair of chunk size and waves) takes the candidate value that times fastest, a
+// candidate having to beat the best by 1 %.  Each trial is the faster
 // of two full runs of the batch, timed with HIP events;
 // schedules change only which spins run where and when, so every trial leaves
 // the batch's outputs as any run does.
@@ -1578,9 +1578,16 @@ SW_API int sw_batch_tune(sw_batch *b, void *stream, uint32_t max_trials, sw_batc
         if ((rc = consider(x))) return rc;
     }
     // The chunk size and the cap interact (a cap counts workgroups of the
-    // chunk's size): one more pass over both from the point reached.
-    if ((rc = chunks())) return rc;
-    if ((rc = caps())) return rc;
+    // chunk's size), so a pass over one and then the other can stop short of
+    // the best pair: every pair is tried from the point reached.
+    {
+        const sw_batch_schedule from = best;
+        for (uint32_t c : {256u, 128u, 64u})
+            for (uint32_t v : {5u, 4u, 3u, 0u}) {
+                sw_batch_schedule x = from; x.chunk_spins = c; x.waves_per_simd = v;
+                if ((rc = consider(x))) return rc;
+            }
+    }
     rc = apply_schedule(e, b, best, s, true);
     if (rc) return rc;
     if (best_out) *best_out = best;
